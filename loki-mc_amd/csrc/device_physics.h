@@ -1,0 +1,551 @@
+// device_physics.h -- per-electron device arithmetic of the Monte Carlo path:
+// Philox variates, analytic free flights, null-collision channel choice and
+// the three collision operators.  One template serves both arithmetic forms:
+//   V = double : plain FP64, the drop-in form; every step follows the
+//                reference's operation order (BoltzmannMC.C:804-1280) so that
+//                channel indices agree bit for bit with a CPU FP64 build.
+//   V = float  : mixed form; velocities, energies, flight and scattering in
+//                FP32, tables in FP32, while the clock, positions and the
+//                energy left after a threshold is subtracted stay in FP64.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "common.h"
+
+namespace ekmc {
+
+// ------------------------------------------------------------------ Philox
+// Philox4x32-10 (Salmon et al., SC'11): ten rounds, Weyl key bumps between.
+__device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
+        uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+    }
+}
+
+__device__ __forceinline__ void philox_call(uint64_t seed, uint64_t stream, uint32_t event,
+                                            uint32_t purpose, uint32_t call, uint32_t w[4]) {
+    w[0] = (uint32_t)stream; w[1] = (uint32_t)(stream >> 32); w[2] = event;
+    w[3] = (purpose << 16) | call;
+    philox10(w, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// Stream of an electron ejected by `parent` at its event `ev`.
+__device__ __forceinline__ uint64_t child_stream(uint64_t seed, uint64_t parent, uint32_t ev) {
+    uint32_t w[4];
+    philox_call(seed, parent, ev, P_CHILD, 0, w);
+    return ((uint64_t)w[1] << 32) | w[0];
+}
+
+// Slot intervals (ekmc.h): 0 (0,1); 1-5,7,10 (0,1]; 6,8,9 [0,1].
+__device__ __forceinline__ int slot_kind(int s) {
+    return s == 0 ? 0 : ((s == 6 || s == 8 || s == 9) ? 2 : 1);
+}
+
+template <typename V> struct EventRng;
+
+// FP64: slot s is the 53-bit integer of words (2h,2h+1) of call s/2.
+template <> struct EventRng<double> {
+    uint64_t seed, stream;
+    uint32_t event;
+    int cached;
+    uint32_t w[4];
+    const double* tape;
+    __device__ EventRng(uint64_t sd, uint64_t st, uint32_t ev, const double* tp = nullptr)
+        : seed(sd), stream(st), event(ev), cached(-1), tape(tp) {}
+    __device__ double u(int s) {
+        if (tape) return tape[s];
+        int call = s >> 1;
+        if (call != cached) { philox_call(seed, stream, event, P_EVENT, (uint32_t)call, w); cached = call; }
+        int h = s & 1;
+        uint64_t x = ((((uint64_t)w[2 * h]) << 32) | w[2 * h + 1]) >> 11;
+        const double k = 1.0 / 9007199254740992.0;
+        int kind = slot_kind(s);
+        return kind == 0 ? ((double)x + 0.5) * k : (kind == 1 ? ((double)x + 1.0) * k : (double)x * k);
+    }
+};
+
+// FP32: slot s is the top 24 bits of word s%4 of call s/4.
+template <> struct EventRng<float> {
+    uint64_t seed, stream;
+    uint32_t event;
+    int cached;
+    uint32_t w[4];
+    const double* tape;
+    __device__ EventRng(uint64_t sd, uint64_t st, uint32_t ev, const double* tp = nullptr)
+        : seed(sd), stream(st), event(ev), cached(-1), tape(tp) {}
+    __device__ float u(int s) {
+        if (tape) return (float)tape[s];
+        int call = s >> 2;
+        if (call != cached) { philox_call(seed, stream, event, P_EVENT, (uint32_t)call, w); cached = call; }
+        float x = (float)(w[s & 3] >> 8);
+        const float k = 1.0f / 16777216.0f;
+        int kind = slot_kind(s);
+        return kind == 0 ? (x + 0.5f) * k : (kind == 1 ? (x + 1.0f) * k : x * k);
+    }
+};
+
+// ---------------------------------------------------------------- electron
+template <typename V> struct Electron {
+    double x, y, z, t;
+    V vx, vy, vz, eps, tfree, nue;
+    uint64_t stream;
+    uint32_t event;
+};
+
+template <typename V> __device__ __forceinline__ V sqr_norm(V a, V b, V c) {
+#pragma clang fp contract(off)
+    return a * a + b * b + c * c;
+}
+
+// electron energy [eV] of a velocity, 0.5*m_e*|v|^2/e in the reference order
+__device__ __forceinline__ double energy_of(double vx, double vy, double vz) {
+#pragma clang fp contract(off)
+    return 0.5 * kMass * sqr_norm(vx, vy, vz) / kCharge;
+}
+__device__ __forceinline__ float energy_of(float vx, float vy, float vz) {
+    return (float)(0.5 * kMass / kCharge) * (vx * vx + vy * vy + vz * vz);
+}
+// FP64 energy of an FP32 velocity (threshold subtraction in the mixed form)
+__device__ __forceinline__ double energy_of_d(float vx, float vy, float vz) {
+    double a = vx, b = vy, c = vz;
+    return 0.5 * kMass * (a * a + b * b + c * c) / kCharge;
+}
+__device__ __forceinline__ double energy_of_d(double vx, double vy, double vz) { return energy_of(vx, vy, vz); }
+
+// sin(x) - x and cos(x) - 1 without cancellation, for the FP32 flights
+__device__ __forceinline__ float sin_minus_x(float x) {
+    if (fabsf(x) < 0.5f) {
+        float x2 = x * x;
+        return -x * x2 * (1.0f / 6.0f) * (1.0f - x2 * (1.0f / 20.0f) * (1.0f - x2 * (1.0f / 42.0f) * (1.0f - x2 * (1.0f / 72.0f))));
+    }
+    return sinf(x) - x;
+}
+__device__ __forceinline__ float cos_minus_1(float x) {
+    float s = sinf(0.5f * x);
+    return -2.0f * s * s;
+}
+
+// --------------------------------------------------------------- flights
+// accelerateElectron (BoltzmannMC.C:804-905), FP64 form in the reference order.
+template <int FC>
+__device__ inline void flight(const CondParams& c, Electron<double>& e, double dt, double& gain) {
+    double prev = e.eps;
+    const double q = kCharge, me = kMass;
+    double v0 = e.vx, v1 = e.vy, v2 = e.vz;
+    const int fc = FC >= 0 ? FC : c.fieldCase;
+    switch (fc) {
+        case FIELD_DC: {
+            e.x += v0 * dt + c.accel[0] * (0.5 * dt * dt);
+            e.y += v1 * dt + c.accel[1] * (0.5 * dt * dt);
+            e.z += v2 * dt + c.accel[2] * (0.5 * dt * dt);
+            e.vx = v0 + c.accel[0] * dt;
+            e.vy = v1 + c.accel[1] * dt;
+            e.vz = v2 + c.accel[2] * dt;
+        } break;
+        case FIELD_AC: {
+            double w = c.omega, phi = w * e.t;
+            double sinPhi = sin(phi), cosPhi = cos(phi);
+            double wDt = w * dt, phase = wDt + phi;
+            double sinPhase = sin(phase), cosPhase = cos(phase);
+            double e_me_w = q / (me * w);
+            double aux1 = e_me_w / w * (cosPhase + wDt * sinPhi - cosPhi);
+            double aux2 = e_me_w * (sinPhi - sinPhase);
+            e.x += v0 * dt + c.E[0] * aux1;
+            e.y += v1 * dt;
+            e.z += v2 * dt + c.E[2] * aux1;
+            e.vx = v0 + c.E[0] * aux2;
+            e.vz = v2 + c.E[2] * aux2;
+        } break;
+        case FIELD_DC_B: {
+            double W = c.cyclotron, WDt = W * dt;
+            double sW = sin(WDt), cW = cos(WDt);
+            double sW_W = sW / W, aux2 = (cW - 1.0) / W;
+            double vEx = q * c.E[0] / (me * W), az = q * c.E[2] / me;
+            e.x += v0 * sW_W + (v1 + vEx) * aux2;
+            e.y += -v0 * aux2 + v1 * sW_W + vEx * (sW_W - dt);
+            e.z += v2 * dt - 0.5 * az * dt * dt;
+            e.vx = v0 * cW - v1 * sW - vEx * sW;
+            e.vy = v0 * sW + v1 * cW + vEx * (cW - 1.0);
+            e.vz = v2 - az * dt;
+        } break;
+        case FIELD_ECR: {
+            double W = c.cyclotron, phi = W * e.t;
+            double sinPhi = sin(phi), cosPhi = cos(phi);
+            double WDt = W * dt, sW = sin(WDt), cW = cos(WDt);
+            double phase = WDt + phi, sinPhase = sin(phase), cosPhase = cos(phase);
+            double cosOpp = cos(phi - WDt);
+            double e_me_W = q / (me * W);
+            double vEx = e_me_W * c.E[0], vEz = e_me_W * c.E[2];
+            double sW_W = sW / W, cWm1_W = (cW - 1.0) / W, cosPhase_W = cosPhase / W;
+            e.x += v0 * sW_W + v1 * cWm1_W - 0.25 * vEx * (cosPhase_W - cosOpp / W + 2.0 * dt * sinPhase);
+            e.y += -v0 * cWm1_W + v1 * sW_W + 0.5 * vEx * (dt * cosPhase_W - 2.0 * cWm1_W * sinPhi - cosPhi * sW_W);
+            e.z += v2 * dt + vEz * (cosPhase_W - cosPhi / W + dt * sinPhi);
+            e.vx = v0 * cW - v1 * sW - 0.5 * vEx * (WDt * cosPhase + cosPhi * sW);
+            e.vy = v0 * sW + v1 * cW - 0.5 * vEx * (WDt * cW * sinPhi + (WDt * cosPhi - sinPhi) * sW);
+            e.vz = v2 + vEz * (sinPhi - sinPhase);
+        } break;
+        default: {
+            double w = c.omega, W = c.cyclotron, phi = w * e.t;
+            double sinPhi = sin(phi), cosPhi = cos(phi);
+            double wDt = w * dt, swD = sin(wDt), cwD = cos(wDt);
+            double phase = wDt + phi, sinPhase = sin(phase), cosPhase = cos(phase);
+            double WDt = W * dt, sW = sin(WDt), cW = cos(WDt);
+            double w2 = w * w, W2 = W * W, d = w2 - W2;
+            double vEx = q / (me * W) * c.E[0], vEz = q / (me * w) * c.E[2];
+            double vEx_d = vEx / d, WvEx_d = W * vEx_d;
+            double sW_W = sW / W, cWm1_W = (cW - 1.0) / W, wSinPhi = w * sinPhi;
+            e.x += v0 * sW_W + v1 * cWm1_W + WvEx_d * (cosPhi * (cwD - cW) + (wSinPhi * sW_W - sinPhi * swD));
+            e.y += -v0 * cWm1_W + v1 * sW_W - vEx_d / w * ((W2 + w2 * cW - w2) * sinPhi + W2 * (w * cosPhi * sW_W - sinPhase));
+            e.z += v2 * dt + vEz * ((cosPhase - cosPhi) / w + dt * sinPhi);
+            e.vx = v0 * cW - v1 * sW + WvEx_d * w * (cW * sinPhi - sinPhase + cosPhi / w * W * sW);
+            e.vy = v0 * sW + v1 * cW + vEx_d * W2 * (cosPhase - cosPhi * cW + wSinPhi * sW_W);
+            e.vz = v2 + vEz * (sinPhi - sinPhase);
+        } break;
+    }
+    e.eps = energy_of(e.vx, e.vy, e.vz);
+    gain = e.eps - prev;
+}
+
+// Mixed form: FP32 velocities and displacements, FP64 clock and positions.
+// The DC, AC and DC+B cases use cancellation-free rewrites of the same
+// closed forms; the two AC+B cases run the FP64 formulas and round.
+template <int FC>
+__device__ inline void flight(const CondParams& c, Electron<float>& e, double dt, double& gain) {
+    float prev = e.eps;
+    float v0 = e.vx, v1 = e.vy, v2 = e.vz;
+    float dtf = (float)dt;
+    const int fc = FC >= 0 ? FC : c.fieldCase;
+    switch (fc) {
+        case FIELD_DC: {
+            float hdt2 = 0.5f * dtf * dtf;
+            float a0 = (float)c.accel[0], a1 = (float)c.accel[1], a2 = (float)c.accel[2];
+            e.x += (double)(v0 * dtf + a0 * hdt2);
+            e.y += (double)(v1 * dtf + a1 * hdt2);
+            e.z += (double)(v2 * dtf + a2 * hdt2);
+            e.vx = v0 + a0 * dtf;
+            e.vy = v1 + a1 * dtf;
+            e.vz = v2 + a2 * dtf;
+        } break;
+        case FIELD_AC: {
+            // phase from the FP64 clock, reduced to [0, 2 pi) before rounding
+            double phid = fmod(c.omega * e.t, 2.0 * kPi);
+            float phi = (float)phid, w = (float)c.omega, d = (float)(c.omega * dt);
+            float sinPhi = sinf(phi), cosPhi = cosf(phi);
+            float sh = sinf(0.5f * d), mid = phi + 0.5f * d;
+            float dS = 2.0f * cosf(mid) * sh;                       // sin(phi+d) - sin(phi)
+            float dmsin = -sin_minus_x(d);                            // d - sin(d)
+            float C1pd = sinPhi * dmsin - 2.0f * cosPhi * sh * sh;    // cos(phi+d)-cos(phi)+d sin(phi)
+            float e_me_w = (float)(kCharge / (kMass * c.omega));
+            float aux1 = e_me_w / w * C1pd;
+            float aux2 = -e_me_w * dS;
+            float E0 = (float)c.E[0], E2 = (float)c.E[2];
+            e.x += (double)(v0 * dtf + E0 * aux1);
+            e.y += (double)(v1 * dtf);
+            e.z += (double)(v2 * dtf + E2 * aux1);
+            e.vx = v0 + E0 * aux2;
+            e.vz = v2 + E2 * aux2;
+        } break;
+        case FIELD_DC_B: {
+            float W = (float)c.cyclotron, x = (float)(c.cyclotron * dt);
+            float sW = sinf(x), cm1 = cos_minus_1(x);
+            float sW_W = sW / W, aux2 = cm1 / W;
+            float vEx = (float)(kCharge * c.E[0] / (kMass * c.cyclotron));
+            float az = (float)(kCharge * c.E[2] / kMass);
+            e.x += (double)(v0 * sW_W + (v1 + vEx) * aux2);
+            e.y += (double)(-v0 * aux2 + v1 * sW_W + vEx * (sin_minus_x(x) / W));
+            e.z += (double)(v2 * dtf - 0.5f * az * dtf * dtf);
+            e.vx = v0 + v0 * cm1 - (v1 + vEx) * sW;
+            e.vy = v0 * sW + v1 + (v1 + vEx) * cm1;
+            e.vz = v2 - az * dtf;
+        } break;
+        default: {
+            Electron<double> d;
+            d.x = e.x; d.y = e.y; d.z = e.z; d.t = e.t;
+            d.vx = v0; d.vy = v1; d.vz = v2; d.eps = prev;
+            double g;
+            flight<FC>(c, d, dt, g);
+            e.x = d.x; e.y = d.y; e.z = d.z;
+            e.vx = (float)d.vx; e.vy = (float)d.vy; e.vz = (float)d.vz;
+        } break;
+    }
+    e.eps = energy_of(e.vx, e.vy, e.vz);
+    gain = (double)e.eps - (double)prev;
+}
+
+// --------------------------------------------------------------- samplers
+// AngularScatteringFunctions.h:27-74; `slot` is the variate slot.
+template <typename V>
+__device__ inline V cos_chi(const DevProcesses& pr, int p, V energy, V energyAfter, EventRng<V>& rng, int slot) {
+    switch (pr.scat[p]) {
+        case 1: return (V)1;  // forward
+        case 2: {             // bornDipole
+            V loss = (V)pr.energyLoss[p];
+            V s = sqrt(energyAfter) + sqrt(energy);
+            V ratio = loss / (s * s);
+            V r2 = ratio * ratio;
+            return (V)1 + (V)2 * r2 / ((V)1 - r2) * ((V)1 - pow(r2, -rng.u(slot)));
+        }
+        case 3:  // surendra
+            return ((V)2 + energy - (V)2 * pow((V)1 + energy, rng.u(slot))) / energy;
+        case 4: {  // coulombScreen
+            V eps = pr.params[2 * p] == 0 ? energy : energyAfter;
+            V sp = (V)pr.params[2 * p + 1] / eps;
+            V R = rng.u(slot);
+            return (sp + (V)1 - ((V)2 * sp + (V)1) * R) / (sp + (V)1 - R);
+        }
+        default: return (V)1 - (V)2 * rng.u(slot);  // isotropic
+    }
+}
+
+template <typename V> struct Sph { V norm, sinT, cosT, sinP, cosP; };
+
+// MathFunctions::cart2sph (MathFunctions.C:143-163)
+template <typename V> __device__ inline bool cart2sph(V x, V y, V z, Sph<V>& s) {
+    V x2My2 = x * x + y * y;
+    V nxy = sqrt(x2My2);
+    s.norm = sqrt(x2My2 + z * z);
+    if (s.norm == (V)0) return false;
+    s.sinT = nxy / s.norm;
+    s.cosT = z / s.norm;
+    if (nxy != (V)0) { s.sinP = y / nxy; s.cosP = x / nxy; }
+    else { s.sinP = (V)1; s.cosP = (V)0; }
+    return true;
+}
+
+// MathFunctions::eulerTransformation (MathFunctions.C:129-141)
+template <typename V>
+__device__ __forceinline__ void euler(V sinChi, V cosChi, V sinEta, V cosEta, const Sph<V>& s, V out[3]) {
+    V sCsE = sinChi * sinEta;
+    V aux = sinChi * cosEta * s.cosT + cosChi * s.sinT;
+    out[0] = -sCsE * s.sinP + aux * s.cosP;
+    out[1] = sCsE * s.cosP + aux * s.sinP;
+    out[2] = -sinChi * cosEta * s.sinT + cosChi * s.cosT;
+}
+
+template <typename V> struct CollOut {
+    int pid;
+    uint32_t flags;  // ST_ZERO_VELOCITY
+    V ejv[3];
+    V ejEps;
+    double dE;
+    bool escape;
+};
+
+// Interpolated table value of two rows (no contraction: bit-exact FP64 form)
+template <typename V> __device__ __forceinline__ V lerp2(V w1, V a, V w2, V b) {
+#pragma clang fp contract(off)
+    return w1 * a + w2 * b;
+}
+
+// Lower-bound search of BoltzmannMC.C:1061-1088 (and the per-gas variant,
+// :986-1010) followed by the zero-rate walk (:1091-1093).
+template <typename V>
+__device__ inline int channel_search(const V* c1, const V* c2, const V* i1, const V* i2, V w1, V w2,
+                                     int L, int R, V target, V prevSum, V ref, V speed, bool perGas) {
+#pragma clang fp contract(off)
+    int chosen = -1;
+    bool broke = false;
+    while (L != R) {
+        int t = (L + R) / 2;
+        V tv = perGas ? prevSum + (w1 * c1[t] + w2 * c2[t] - ref) * speed : w1 * c1[t] + w2 * c2[t];
+        if (target < tv) R = t;
+        else if (target > tv) L = t + 1;
+        else { chosen = t; broke = true; break; }
+    }
+    if (!broke) chosen = L;
+    while (chosen > 0 && w1 * i1[chosen] == (V)0 && w2 * i2[chosen] == (V)0) --chosen;
+    return chosen;
+}
+
+// performCollision (BoltzmannMC.C:907-1113) with conservativeCollision
+// (:1115-1193), ionizationCollision (:1195-1272) and attachmentCollision
+// (:1274-1280).  `tab` points at this condition's tables: cumul [I*P],
+// interp [I*P], nutot [I].
+template <typename V, bool THERMAL>
+__device__ inline void collide(const CondParams& c, const DevProcesses& pr, const V* cumul, const V* interp,
+                               const V* nutot, Electron<V>& e, EventRng<V>& rng, CollOut<V>& o) {
+    const int P = pr.P, I = c.I;
+    o.pid = -1; o.flags = 0; o.ejEps = (V)0; o.dE = 0; o.escape = false;
+    o.ejv[0] = o.ejv[1] = o.ejv[2] = (V)0;
+    V Vt[3] = {(V)0, (V)0, (V)0};
+    int chosen = -1;
+    const int gte = pr.gasTempEffect;
+    const bool thermal = THERMAL && (gte == 1 || (gte == 2 && (double)e.eps < 20.0 * c.gasEnergy));
+    const V step = (V)c.step;
+    if (THERMAL && thermal) {
+        // MathFunctions::unitNormalRand3 (MathFunctions.C:54-59)
+        V r1 = rng.u(1), r2 = rng.u(2), r3 = rng.u(3), r4 = rng.u(4);
+        V aux1 = sqrt((V)-2 * log(r1)), aux2 = (V)(2.0 * kPi) * r2;
+        V vers[3] = {aux1 * cos(aux2), aux1 * sin(aux2), sqrt((V)-2 * log(r3)) * cos((V)(2.0 * kPi) * r4)};
+        V randCS = e.nue * rng.u(5) / (V)c.N;
+        V prevSum = (V)0;
+        for (int g = 0; g < pr.G && chosen == -1; ++g) {
+            if (pr.gasFrac[g] == 0) continue;
+            int L = pr.gasFirst[g], R = pr.gasLast[g];
+            V sd = (V)pr.thermalStd[L];
+            for (int k = 0; k < 3; ++k) Vt[k] = vers[k] * sd;
+            V dx = e.vx - Vt[0], dy = e.vy - Vt[1], dz = e.vz - Vt[2];
+            V rel = sqrt(sqr_norm(dx, dy, dz));
+            V over = (V)0.5 * (V)pr.rMass[L] * rel * rel / (V)kCharge / step;
+            int k1 = (int)fmin(over, (V)(I - 1));
+            int k2 = (int)fmin((V)(k1 + 1), (V)(I - 1));
+            V w1 = (V)k2 - over;
+            w1 = w1 < (V)0 ? (V)0 : (V)1;
+            V w2 = (V)1 - w1;
+            const V* c1 = cumul + (size_t)k1 * P;
+            const V* c2 = cumul + (size_t)k2 * P;
+            const V* i1 = interp + (size_t)k1 * P;
+            const V* i2 = interp + (size_t)k2 * P;
+            V ref = (V)0;
+            if (L > 0) ref = lerp2(w1, c1[L - 1], w2, c2[L - 1]);
+            V lim = prevSum + (lerp2(w1, c1[R], w2, c2[R]) - ref) * rel;
+            if (randCS > lim) { prevSum = lim; continue; }
+            chosen = channel_search(c1, c2, i1, i2, w1, w2, L, R, randCS, prevSum, ref, rel, true);
+        }
+        if (chosen == -1) return;
+    } else {
+        V randF = e.nue * rng.u(5);
+        V over = e.eps / step;
+        int k1 = (int)fmin(over, (V)(I - 1));
+        int k2 = (int)fmin((V)(k1 + 1), (V)(I - 1));
+        V w1 = (V)k2 - over;
+        if (w1 < (V)0) w1 = (V)0;
+        V w2 = (V)1 - w1;
+        V tot = lerp2(w1, nutot[k1], w2, nutot[k2]);
+        o.escape = e.nue < tot;
+        if (randF > tot) return;
+        const V* c1 = cumul + (size_t)k1 * P;
+        const V* c2 = cumul + (size_t)k2 * P;
+        const V* i1 = interp + (size_t)k1 * P;
+        const V* i2 = interp + (size_t)k2 * P;
+        V speed = sqrt(sqr_norm(e.vx, e.vy, e.vz));
+        V randCS = randF / (V)c.N / speed;
+        chosen = channel_search(c1, c2, i1, i2, w1, w2, 0, P - 1, randCS, (V)0, (V)0, speed, false);
+    }
+    o.pid = chosen;
+    const int type = pr.type[chosen];
+    const V inc = e.eps;
+    if (type == 0) {
+        // conservativeCollision
+        double M = pr.tMass[chosen], mu = pr.rMass[chosen], loss = pr.energyLoss[chosen];
+        if (THERMAL && thermal) {
+            Sph<V> s;
+            if (!cart2sph(e.vx - Vt[0], e.vy - Vt[1], e.vz - Vt[2], s)) { o.flags |= ST_ZERO_VELOCITY; o.pid = -1; return; }
+            double relE = 0.5 * mu * (double)s.norm * (double)s.norm / kCharge;
+            double relAfter = relE - loss;
+            if (relAfter <= 0) { o.pid = -1; return; }
+            V cc = cos_chi<V>(pr, chosen, (V)relE, (V)relAfter, rng, 6);
+            V sc = sqrt((V)1 - cc * cc);
+            V eta = (V)(2.0 * kPi) * rng.u(7);
+            V u[3];
+            euler(sc, cc, (V)sin(eta), (V)cos(eta), s, u);
+            double rel2 = (double)s.norm * (double)s.norm - 2.0 / mu * loss * kCharge;
+            if (!(rel2 > 0)) { o.pid = -1; return; }  // rounding at the threshold edge
+            V sp = (V)sqrt(rel2);
+            V fM = (V)(M / (kMass + M)), fe = (V)(kMass / (kMass + M));
+            e.vx = fM * (sp * u[0]) + (fe * e.vx + fM * Vt[0]);
+            e.vy = fM * (sp * u[1]) + (fe * e.vy + fM * Vt[1]);
+            e.vz = fM * (sp * u[2]) + (fe * e.vz + fM * Vt[2]);
+        } else {
+            Sph<V> s;
+            if (!cart2sph(e.vx, e.vy, e.vz, s)) { o.flags |= ST_ZERO_VELOCITY; o.pid = -1; return; }
+            double after = (double)inc - loss;
+            if (after <= 0) { o.pid = -1; return; }
+            V cc = cos_chi<V>(pr, chosen, inc, (V)after, rng, 6);
+            V sc = sqrt((V)1 - cc * cc);
+            V eta = (V)(2.0 * kPi) * rng.u(7);
+            V u[3];
+            euler(sc, cc, (V)sin(eta), (V)cos(eta), s, u);
+            double v2after = (double)s.norm * (double)s.norm - 2.0 / kMass * loss * kCharge;
+            if (!(v2after > 0)) { o.pid = -1; return; }  // rounding at the threshold edge
+            V sp = sqrt((V)v2after * ((V)1 - (V)(2.0 * mu / (kMass + M)) * ((V)1 - cc)));
+            e.vx = sp * u[0]; e.vy = sp * u[1]; e.vz = sp * u[2];
+        }
+        e.eps = energy_of(e.vx, e.vy, e.vz);
+        o.dE = (double)e.eps - (double)inc;
+    } else if (type == 1) {
+        // ionizationCollision
+        double thr = pr.energyLoss[chosen];
+        if ((double)inc < thr) { o.pid = -1; return; }
+        Sph<V> s;
+        if (!cart2sph(e.vx, e.vy, e.vz, s)) { o.flags |= ST_ZERO_VELOCITY; o.pid = -1; return; }
+        double netd = (double)inc - thr;
+        V net = (V)netd, ej;
+        if (pr.ionizOp == 2) {
+            V w = (V)pr.wParam[chosen];
+            ej = w * tan(rng.u(8) * atan(net / ((V)2 * w)));
+        } else if (pr.ionizOp == 3) {
+            ej = rng.u(8) * net;
+        } else {
+            ej = (V)pr.sharingFactor * net;
+        }
+        V sc_e = net - ej;
+        V sCs, cCs, sCe, cCe, sEs, cEs, sEe, cEe;
+        if (pr.scat[chosen] == 5) {
+            cCs = sqrt(sc_e / net); sCs = sqrt((V)1 - cCs * cCs);
+            V eta = (V)(2.0 * kPi) * rng.u(7);
+            sEs = sin(eta); cEs = cos(eta);
+            cCe = sqrt(ej / net); sCe = sqrt((V)1 - cCe * cCe);
+            sEe = -sEs; cEe = -cEs;
+        } else {
+            cCs = cos_chi<V>(pr, chosen, inc, sc_e, rng, 6);
+            sCs = sqrt((V)1 - cCs * cCs);
+            V eta = (V)(2.0 * kPi) * rng.u(7);
+            sEs = sin(eta); cEs = cos(eta);
+            cCe = cos_chi<V>(pr, chosen, inc, ej, rng, 9);
+            sCe = sqrt((V)1 - cCe * cCe);
+            V etaE = (V)(2.0 * kPi) * rng.u(10);
+            sEe = sin(etaE); cEe = cos(etaE);
+        }
+        V u[3];
+        euler(sCs, cCs, sEs, cEs, s, u);
+        V sp = sqrt((V)2 * sc_e * (V)kCharge / (V)kMass);
+        e.vx = sp * u[0]; e.vy = sp * u[1]; e.vz = sp * u[2];
+        euler(sCe, cCe, sEe, cEe, s, u);
+        V spE = sqrt((V)2 * ej * (V)kCharge / (V)kMass);
+        o.ejv[0] = spE * u[0]; o.ejv[1] = spE * u[1]; o.ejv[2] = spE * u[2];
+        o.ejEps = ej;
+        e.eps = sc_e;
+        o.dE = -thr;
+    } else {
+        o.dE = -(double)inc;  // attachment
+    }
+}
+
+// One synchronization-interval step of an electron
+// (BoltzmannMC.C:637-680): draw a free time if none is pending, then fly
+// either to t_sync (partial flight, returns true = done) or to the collision.
+// Returns: 0 = collision happened (o filled), 1 = reached t_sync.
+template <typename V, int FC, bool THERMAL>
+__device__ __forceinline__ int interval_step(const CondParams& c, const DevProcesses& pr, const V* cumul,
+                                             const V* interp, const V* nutot, Electron<V>& e,
+                                             CollOut<V>& o, double& gain) {
+    if (e.t == c.tsync) return 1;
+    if (e.tfree < (V)0) {  // pending free time undefined (NON_DEF is negative)
+        EventRng<V> r0(c.seed, e.stream, e.event);
+        e.tfree = -log(r0.u(0)) / (V)c.trial;
+        e.nue = (V)c.trial;
+    }
+    if (e.t + (double)e.tfree > c.tsync) {
+        double dt = c.tsync - e.t;
+        flight<FC>(c, e, dt, gain);
+        e.t = c.tsync;
+        e.tfree = (V)((double)e.tfree - dt);
+        return 1;
+    }
+    flight<FC>(c, e, (double)e.tfree, gain);
+    e.t += (double)e.tfree;
+    EventRng<V> rng(c.seed, e.stream, e.event);
+    collide<V, THERMAL>(c, pr, cumul, interp, nutot, e, rng, o);
+    e.event += 1;
+    EventRng<V> rn(c.seed, e.stream, e.event);
+    e.tfree = -log(rn.u(0)) / (V)c.trial;
+    e.nue = (V)c.trial;
+    return 0;
+}
+
+}  // namespace ekmc

@@ -1,0 +1,245 @@
+"""Deterministic generator of the synthetic LXCat cross-section sets and
+working-condition lists used by the tests and the benchmark.
+
+Run as a script to (re)write the files next to it:
+
+    python loki-mc_amd/gases/make_gases.py
+
+Every file it writes is committed; the generator exists so the data can be
+audited and regenerated.  The LXCat text follows the block layout that
+LoKI-MC's reader accepts (Parse.C:43-125, 127-295): a ``PROCESS:`` line, the
+``PARAM.: E = x eV`` threshold line right after it, a ``COMMENT: [...]``
+description line whose last comma field is the collision type (optionally
+followed by ``integral`` / ``momentum-transfer``), and a two-column table
+between ``-----`` rulers.
+
+Gas sets (sizes as in the workload descriptions):
+
+* ``atomic40``  -- 40 amu atomic gas, 4 channels (elastic, 2 excitations,
+  ionization), isotropic.  One DC condition at 100 Td.
+* ``n2like``    -- 28 amu molecule, 18 channels, screened-Coulomb anisotropic
+  scattering with incident-energy-dependent screening.
+* ``o2like``    -- 32 amu molecule, 22 channels incl. dissociative attachment
+  (4.4 eV threshold, 1e-22 m^2 peak) and ionization at 12.1 eV.
+* ``mix5``      -- 5 gases, 200 channels (150 excitation, 5 ionization,
+  3 attachment, 42 elastic/rotational).
+"""
+import json
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+AMU = 1.660539040e-27            # Constant.h:9
+BOLTZMANN = 1.38064852e-23       # Constant.h:6
+GRID = np.logspace(-3.0, 3.0, 1000)   # 1000-point log grid, 1e-3 .. 1000 eV
+
+
+def _fmt(x):
+    return "%.6e" % x
+
+
+def _log_form(amp, eth):
+    """sigma = amp * ln(E/Eth)/(E/Eth) on the grid points above Eth, with the
+    threshold point itself (sigma = 0) first."""
+    e = [eth] + [float(x) for x in GRID if x > eth * (1 + 1e-12)]
+    s = [0.0] + [amp * math.log(x / eth) / (x / eth) for x in e[1:]]
+    return e, s
+
+
+def _resonant_form(amp, eth, centre, width, tail):
+    """Threshold point, then a Gaussian resonance plus a small log tail."""
+    e = [eth] + [float(x) for x in GRID if x > eth * (1 + 1e-12)]
+    s = [0.0]
+    for x in e[1:]:
+        g = amp * math.exp(-((x - centre) / width) ** 2)
+        s.append(g + tail * math.log(x / eth) / (x / eth))
+    return e, s
+
+
+class GasWriter:
+    def __init__(self, name, mass_amu):
+        self.name = name
+        self.mass_amu = mass_amu
+        self.blocks = []
+        self.processes = []   # metadata mirrored into the json sidecar
+
+    def add(self, kind, product, threshold, energies, sigmas, scattering="isotropic",
+            params=(), subtype=None):
+        target = "%s(X)" % self.name
+        if kind == "Ionization":
+            rhs = "E + E + %s(+,X)" % self.name
+        elif kind == "Attachment":
+            rhs = "%s(-,X)" % self.name
+        else:
+            rhs = "E + %s(%s)" % (self.name, product)
+        desc = "E + %s -> %s, %s" % (target, rhs, kind)
+        if subtype:
+            desc += ", " + subtype
+        lines = ["SPECIES: e / %s" % self.name,
+                 "PROCESS: %s" % desc,
+                 "PARAM.:  E = %.6f eV, complete set" % threshold,
+                 "COMMENT: [%s]" % desc,
+                 "UPDATED: 2026-01-01 00:00:00",
+                 "COLUMNS: Energy (eV) | Cross section (m2)",
+                 "-----------------------------"]
+        lines += ["%s %s" % (_fmt(e), _fmt(s)) for e, s in zip(energies, sigmas)]
+        lines += ["-----------------------------", ""]
+        self.blocks.append("\n".join(lines))
+        self.processes.append({"type": kind, "threshold": threshold,
+                               "scattering": scattering, "params": list(params)})
+
+    def text(self):
+        return "\n".join(self.blocks)
+
+
+def atomic40():
+    g = GasWriter("A40", 40.0)
+    e = [float(x) for x in GRID]
+    g.add("Elastic", "X", 0.0, e, [1e-20 * (0.1 + x / (1 + x)) for x in e])
+    for eth in (11.5, 13.0):
+        g.add("Excitation", "exc%g" % eth, eth, *_log_form(2e-21, eth))
+    g.add("Ionization", "+", 15.8, *_log_form(3e-21, 15.8))
+    return [g]
+
+
+def n2like(name="N2L", screen=("coulombScreen", (0.0, 1.0))):
+    g = GasWriter(name, 28.0)
+    e = [float(x) for x in GRID]
+    el = [1e-19 * x / (1 + x ** 1.2) + 1.2e-19 * math.exp(-((x - 2.3) / 0.5) ** 2) + 5e-21
+          for x in e]
+    scat, prm = screen
+    g.add("Elastic", "X", 0.0, e, el, scattering=scat, params=prm, subtype="integral")
+    for k in range(1, 11):
+        eth = 0.29 * k
+        g.add("Vibrational", "v%d" % k, eth,
+              *_resonant_form(4e-20 / k, eth, 2.3 + 0.05 * k, 0.45, 3e-22 / k))
+    for i, eth in enumerate(np.linspace(6.2, 12.5, 6)):
+        g.add("Excitation", "exc%d" % i, float(eth), *_log_form(1.2e-20, float(eth)),
+              scattering=scat, params=prm)
+    g.add("Ionization", "+", 15.6, *_log_form(6e-20, 15.6))
+    return g
+
+
+def o2like(name="O2L"):
+    g = GasWriter(name, 32.0)
+    e = [float(x) for x in GRID]
+    el = [6e-20 * (0.15 + x / (1 + 0.8 * x)) / (1 + 0.02 * x) + 4e-21 for x in e]
+    g.add("Elastic", "X", 0.0, e, el)
+    for k in range(1, 5):
+        eth = 0.193 * k
+        g.add("Vibrational", "v%d" % k, eth,
+              *_resonant_form(2e-21 / k, eth, 0.6 + 0.1 * k, 0.3, 1e-22 / k))
+    for eth, amp in ((0.977, 6e-22), (1.627, 2e-22)):
+        g.add("Excitation", "low%g" % eth, eth, *_log_form(amp, eth))
+    # dissociative attachment: peak 1e-22 m^2 near 6.5 eV
+    ea = [4.4] + [float(x) for x in GRID if 4.4 < x < 30.0] + [30.0]
+    sa = [0.0] + [1e-22 * math.exp(-((x - 6.5) / 1.2) ** 2) for x in ea[1:-1]] + [0.0]
+    g.add("Attachment", "-", 4.4, ea, sa)
+    for i, eth in enumerate(np.linspace(4.5, 11.5, 13)):
+        g.add("Excitation", "exc%d" % i, float(eth), *_log_form(6e-21, float(eth)))
+    g.add("Ionization", "+", 12.1, *_log_form(6e-20, 12.1))
+    return g
+
+
+def mix5():
+    gases = []
+    specs = [("G28", 28.0, True), ("G32", 32.0, True), ("G44", 44.0, True),
+             ("G40", 40.0, False), ("G04", 4.0, False)]
+    for gi, (name, mass, has_att) in enumerate(specs):
+        g = GasWriter(name, mass)
+        e = [float(x) for x in GRID]
+        el = [(3e-20 + 2e-21 * gi) * (0.2 + x / (1 + x)) / (1 + 0.01 * x) + 3e-21 for x in e]
+        g.add("Elastic", "X", 0.0, e, el)
+        n_rot = 7 if has_att else 8
+        for r in range(n_rot):
+            eth = 0.002 * (r + 1) * (1 + 0.3 * gi)
+            g.add("Rotational", "J%d" % r, eth, *_log_form(2e-21 / (r + 1), eth))
+        for k, eth in enumerate(np.linspace(0.3 + 0.2 * gi, 14.0 + 0.5 * gi, 30)):
+            g.add("Excitation", "x%d" % k, float(eth), *_log_form(2.5e-21, float(eth)))
+        if has_att:
+            eth = 2.0 + 1.5 * gi
+            ea = [eth] + [float(x) for x in GRID if eth < x < eth + 20.0] + [eth + 20.0]
+            sa = [0.0] + [8e-23 * math.exp(-((x - eth - 2.0) / 1.0) ** 2)
+                          for x in ea[1:-1]] + [0.0]
+            g.add("Attachment", "-", eth, ea, sa)
+        g.add("Ionization", "+", 12.0 + 1.2 * gi, *_log_form(5e-20, 12.0 + 1.2 * gi))
+        gases.append(g)
+    return gases
+
+
+def _condition(en, angle=180.0, freq=0.0, bn=0.0, tg=300.0, p=133.32, n_e=10000):
+    return {"reduced_elec_field": en, "elec_field_angle": angle,
+            "excitation_frequency": freq, "reduced_mag_field": bn,
+            "gas_temperature": tg, "gas_density": p / (BOLTZMANN * tg), "n_electrons": n_e}
+
+
+def workloads():
+    """The five workloads: gas sets, fractions and working conditions."""
+    rng = np.random.default_rng(20260101)
+    out = {}
+    out["cfg1_atomic40_dc"] = {
+        "gases": ["atomic40"], "fractions": [1.0], "gas_temperature_effect": "false",
+        "ionization_operator": "randomUniform", "initial_te_over_tg": 38.6817,
+        "max_collisions_before_ss": 100, "max_collisions_after_ss": 1000,
+        "conditions": [_condition(100.0, n_e=10000)]}
+    out["cfg2_n2like_scan"] = {
+        "gases": ["n2like"], "fractions": [1.0], "gas_temperature_effect": "false",
+        "ionization_operator": "equalSharing", "initial_te_over_tg": 38.6817,
+        "conditions": [_condition(float(en), n_e=100000)
+                       for en in np.logspace(1.0, 3.0, 8)]}
+    # 64 conditions: 22 + 21 + 21 log-spaced E/N values in 1..1000 Td per B/N
+    conds = []
+    for bn, n_en in ((0.0, 22), (200.0, 21), (1000.0, 21)):
+        for en in np.logspace(0.0, 3.0, n_en):
+            conds.append((float(en), bn))
+    counts = np.exp(rng.uniform(np.log(1e4), np.log(1e6), size=len(conds))).astype(int)
+    out["cfg3_o2like_batched"] = {
+        "gases": ["o2like"], "fractions": [1.0], "gas_temperature_effect": "false",
+        "ionization_operator": "equalSharing", "initial_te_over_tg": 38.6817,
+        "conditions": [_condition(en, angle=90.0, bn=bn, n_e=int(c))
+                       for (en, bn), c in zip(conds, counts)]}
+    conds4 = []
+    p, tg = 133.32, 300.0
+    n_gas = p / (BOLTZMANN * tg)
+    for erms in (50.0, 100.0, 200.0, 400.0):
+        for wn in np.logspace(-16.0, -13.0, 6):
+            freq = float(wn) * n_gas / (2.0 * math.pi)
+            conds4.append(_condition(erms, freq=freq, n_e=200000))
+    out["cfg4_air_ac"] = {
+        "gases": ["n2like", "o2like"], "fractions": [0.79, 0.21],
+        "gas_temperature_effect": "false", "ionization_operator": "equalSharing",
+        "initial_te_over_tg": 38.6817, "n_integration_phases": 32, "n_energy_cells": 400,
+        "conditions": conds4}
+    out["cfg5_mix5_large"] = {
+        "gases": ["mix5"], "fractions": [0.4, 0.2, 0.15, 0.15, 0.1],
+        "gas_temperature_effect": "true", "ionization_operator": "equalSharing",
+        "initial_te_over_tg": 11.6045,
+        "conditions": [_condition(300.0, angle=90.0, bn=500.0, tg=1000.0,
+                                  n_e=100000000)]}
+    return out
+
+
+def gas_sets():
+    n2 = n2like()
+    return {"atomic40": atomic40(), "n2like": [n2], "o2like": [o2like()], "mix5": mix5()}
+
+
+def main():
+    sets = gas_sets()
+    meta = {}
+    for set_name, gases in sets.items():
+        path = os.path.join(HERE, "%s_LXCat.txt" % set_name)
+        with open(path, "w") as f:
+            f.write("\n".join(g.text() for g in gases))
+        meta[set_name] = [{"name": g.name, "mass_amu": g.mass_amu,
+                           "processes": g.processes} for g in gases]
+    with open(os.path.join(HERE, "gases.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    with open(os.path.join(HERE, "workloads.json"), "w") as f:
+        json.dump(workloads(), f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,253 @@
+"""GPU parity of the engine (through the C ABI) against the CPU oracle.
+
+Event level (identical uniform variates): channel indices bit-exact, post-
+collision and post-flight velocities within 1e-12 relative in FP64.
+Trajectory level: one synchronization interval of a whole ensemble, same
+Philox streams, no population control involved.
+Run level (independent seeds): swarm quantities within combined errors.
+"""
+import numpy as np
+import pytest
+
+import ekmc
+import oracle_py as O
+from helpers import condition, mixture, speeds, unit_vectors
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(mix, conds, num):
+    return ekmc.Engine(mix, conds, num)
+
+
+KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+]
+
+
+def test_device_philox_known_answers_and_oracle():
+    ctr = np.array([k[0] for k in KAT], np.uint32)
+    key = np.array([k[1] for k in KAT], np.uint32)
+    assert ekmc.philox(ctr, key).tolist() == [list(k[2]) for k in KAT]
+    rng = np.random.default_rng(0)
+    c = rng.integers(0, 2 ** 32, (5000, 4), dtype=np.uint64).astype(np.uint32)
+    k = rng.integers(0, 2 ** 32, (5000, 2), dtype=np.uint64).astype(np.uint32)
+    assert np.array_equal(ekmc.philox(c, k), O.philox(c, k))
+
+
+COLLIDE_CASES = [
+    ("cfg1_atomic40_dc", "randomUniform", "false"),
+    ("cfg2_n2like_scan", "equalSharing", "false"),
+    ("cfg3_o2like_batched", "usingSDCS", "false"),
+    ("cfg3_o2like_batched", "oneTakesAll", "true"),
+    ("cfg5_mix5_large", "equalSharing", "true"),
+    ("cfg5_mix5_large", "randomUniform", "smartActivation"),
+]
+
+
+@pytest.mark.parametrize("wl,ioniz,gastemp", COLLIDE_CASES)
+def test_collisions_match_oracle_with_identical_variates(wl, ioniz, gastemp):
+    mix, conds, num = ekmc.workload(wl, n_electrons=1024, conditions=[0])
+    num.ionization_operator = ekmc.IONIZ[ioniz]
+    num.gas_temperature_effect = ekmc.GASTEMP[gastemp]
+    num.n_interp_points = 4000
+    eng = _engine(mix, conds, num)
+    m = O.Model(mix, conds[0], num)
+    maxE = 90.0
+    _, _, tot, mx, _ = m.tables(maxE)
+    rng = np.random.default_rng(11)
+    n = 40000
+    e = np.concatenate([rng.uniform(0.0005, 0.2, n // 4), rng.uniform(0.2, 95.0, n - n // 4)])
+    v = unit_vectors(rng, n) * speeds(e)[:, None]
+    e = 0.5 * ekmc.EMASS * (v * v).sum(1) / ekmc.CHARGE
+    nue = np.full(n, mx[-1] * 1.05)
+    var = rng.uniform(0, 1, (n, ekmc.VARIATES_PER_EVENT))
+    var[:, [0, 1, 2, 3, 4, 5, 7, 10]] = np.maximum(var[:, [0, 1, 2, 3, 4, 5, 7, 10]], 1e-300)
+    pid_g, v_g, e_g, ejv_g, eje_g, de_g = eng.op_collide(0, maxE, v, e, nue, var)
+    st, pid_o, v_o, e_o, ejv_o, eje_o, de_o = m.collide(v, e, nue, var)
+    assert st == 0
+    assert np.array_equal(pid_g, pid_o)
+    assert (pid_g >= 0).sum() > n // 20
+    np.testing.assert_allclose(v_g, v_o, rtol=1e-12, atol=1e-12 * np.abs(v_o).max())
+    np.testing.assert_allclose(e_g, e_o, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(ejv_g, ejv_o, rtol=1e-12, atol=1e-12 * max(np.abs(ejv_o).max(), 1.0))
+    np.testing.assert_allclose(eje_g, eje_o, rtol=1e-12)
+    np.testing.assert_allclose(de_g, de_o, rtol=1e-11, atol=1e-12)
+    eng.close()
+
+
+@pytest.mark.parametrize("case", ["dc", "ac", "dcb", "acb", "ecr"])
+def test_flights_match_oracle(case):
+    mix, _, num = ekmc.workload("cfg1_atomic40_dc")
+    N = 3.22e22
+    if case == "dc":
+        cond = condition(en=150.0, angle=60.0)
+    elif case == "ac":
+        cond = condition(en=150.0, angle=60.0, freq=3e9)
+    elif case == "dcb":
+        cond = condition(en=150.0, angle=90.0, bn=800.0)
+    elif case == "acb":
+        cond = condition(en=150.0, angle=45.0, freq=2e9, bn=800.0)
+    else:
+        W = ekmc.CHARGE * 800e-27 * N / ekmc.EMASS
+        cond = condition(en=150.0, angle=45.0, freq=W / (2 * np.pi), bn=800.0)
+    cond.gas_density = N
+    cond.n_electrons = 256
+    eng = _engine(mix, [cond], num)
+    m = O.Model(mix, cond, num)
+    rng = np.random.default_rng(12)
+    n = 20000
+    t0 = rng.uniform(0, 5e-8, n)
+    dt = rng.uniform(1e-13, 3e-10, n)
+    v0 = unit_vectors(rng, n) * speeds(rng.uniform(0.01, 40, n))[:, None]
+    r0 = rng.normal(size=(n, 3)) * 1e-2
+    pg, vg, eg, gg = eng.op_accelerate(0, t0, dt, r0, v0)
+    po, vo, eo, go = m.accelerate(t0, dt, r0, v0)
+    np.testing.assert_allclose(vg, vo, rtol=1e-12, atol=1e-12 * np.abs(vo).max())
+    np.testing.assert_allclose(pg, po, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(eg, eo, rtol=1e-12)
+    np.testing.assert_allclose(gg, go, rtol=1e-9, atol=1e-12)
+    eng.close()
+
+
+def _no_ionization_gas():
+    e = np.logspace(-3, 3, 400)
+    procs = [dict(type=0, elastic=True, energy=e, sigma=1e-20 * (0.3 + e / (1 + e)), mass_amu=20.0)]
+    for thr in (1.2, 4.0, 9.0):
+        ee = e[e >= thr]
+        procs.append(dict(type=0, threshold=thr, energy=ee, sigma=4e-21 * np.log(ee / thr) / (ee / thr),
+                          mass_amu=20.0))
+    return mixture(procs)
+
+
+def test_initial_ensemble_and_one_interval_match_oracle():
+    """Maxwellian start and one synchronization interval of 8192 electrons:
+    same Philox streams, so every electron follows the same flights."""
+    mix = _no_ionization_gas()
+    cond = condition(en=120.0, n_e=8192)
+    num = ekmc.numerics_default()
+    num.initial_elec_temp_over_gas_temp = 38.68
+    num.seed = 1234
+    eng = _engine(mix, [cond], num)
+    eng.initialize()
+    pos0, vel0, e0, t0 = eng.electrons(0)
+    assert np.all(pos0 == 0) and np.all(t0 == 0)
+    eng.step(1)
+    st = eng.status(0)
+    pos1, vel1, e1, t1 = eng.electrons(0)
+    tsync = st.time
+    trial = st.trial_collision_frequency
+    assert np.all(t1 == tsync)
+    m = O.Model(mix, cond, num)
+    n = cond.n_electrons
+    state = dict(pos=pos0, vel=vel0, energy=e0, time=t0, free_time=np.full(n, ekmc.NON_DEF, float),
+                 nu_e=np.full(n, trial), stream=np.arange(n, dtype=np.uint64),
+                 event=np.zeros(n, np.uint32))
+    pool, counters, nnull = m.advance_pool(num.seed, trial, tsync, state, n)
+    assert len(pool["energy"]) == n and pool["alive"].all()
+    np.testing.assert_allclose(vel1, pool["vel"], rtol=1e-10, atol=1e-9 * np.abs(vel1).max())
+    np.testing.assert_allclose(pos1, pool["pos"], rtol=1e-9, atol=1e-9 * np.abs(pos1).max())
+    ev, real = eng.event_count()
+    assert real == counters.sum() and ev == real + nnull
+    eng.close()
+
+
+def _short_numerics(seed, precision=ekmc.FP64):
+    num = ekmc.numerics_default()
+    num.precision = precision
+    num.seed = seed
+    num.initial_elec_temp_over_gas_temp = 38.6817
+    num.max_collisions_before_ss = 30
+    num.n_integration_points = 600
+    return num
+
+
+def test_runs_are_bitwise_reproducible():
+    mix, conds, _ = ekmc.workload("cfg3_o2like_batched", n_electrons=4000, conditions=[5, 30, 60])
+    out = []
+    for _ in range(2):
+        num = _short_numerics(99)
+        eng = _engine(mix, conds, num)
+        eng.initialize()
+        eng.step(300)
+        out.append([eng.series(c)["mean_energy"].copy() for c in range(3)] +
+                   [eng.electrons(c)[1].copy() for c in range(3)])
+        eng.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("wl,idx", [("cfg3_o2like_batched", [21]), ("cfg3_o2like_batched", [3])])
+def test_population_control_keeps_n_and_balances_power(wl, idx):
+    """21: 1000 Td, ionization-dominated growth; 3: 2.7 Td, attachment-dominated."""
+    mix, conds, _ = ekmc.workload(wl, n_electrons=20000, conditions=idx)
+    num = _short_numerics(5)
+    eng = _engine(mix, conds, num)
+    eng.solve()
+    r = eng.results(0)
+    pos, vel, e, t = eng.electrons(0)
+    assert np.all(np.isfinite(e)) and np.all(t == t[0])
+    assert len(np.unique(np.round(vel[:, 0], 3))) > 0.9 * len(vel)  # no mass duplication
+    rates, counts, gain, loss = eng.process_data(0)
+    if idx == [21]:
+        assert r.total_ion_rate_coeff > r.total_att_rate_coeff and r.power_growth < 0
+    else:
+        assert r.total_att_rate_coeff > r.total_ion_rate_coeff and r.power_growth > 0
+    assert r.power_balance_rel_error < 0.02
+    eng.close()
+
+
+def _combined_sigma(a, ea, b, eb, k=4.0):
+    return abs(a - b) <= k * np.hypot(ea, eb)
+
+
+@pytest.mark.slow
+def test_cfg1_swarm_statistics_match_oracle():
+    """Independent seeds: GPU FP64 engine vs the CPU oracle on the 40 amu gas
+    at 100 Td, mean energy / drift / diffusion within combined errors."""
+    mix, conds, num = ekmc.workload("cfg1_atomic40_dc")
+    num.n_integration_points = 2000
+    num.max_collisions_after_ss = 1e100
+    num.seed = 17
+    eng = _engine(mix, conds, num)
+    eng.solve()
+    g = eng.results(0)
+    num2 = ekmc.workload("cfg1_atomic40_dc")[2]
+    num2.n_integration_points = 2000
+    num2.max_collisions_after_ss = 1e100
+    num2.seed = 18
+    o, rates_o, _, _ = O.Model(mix, conds[0], num2).solve(n_threads=8)
+    assert _combined_sigma(g.mean_energy, g.mean_energy_error, o.mean_energy, o.mean_energy_error)
+    assert _combined_sigma(g.flux_drift_velocity[2], g.flux_drift_velocity_error[2],
+                           o.flux_drift_velocity[2], o.flux_drift_velocity_error[2])
+    for k in (0, 4, 8):
+        assert _combined_sigma(g.flux_diffusion[k], g.flux_diffusion_error[k],
+                               o.flux_diffusion[k], o.flux_diffusion_error[k], k=5.0)
+    rates_g = eng.process_data(0)[0]
+    # rate coefficients: Poisson errors of the counts
+    _, counts, _, _ = eng.process_data(0)
+    for p in range(len(rates_g)):
+        if counts[p] > 100:
+            rel = 1 / np.sqrt(counts[p])
+            assert abs(rates_g[p] / rates_o[p] - 1) < 6 * rel + 0.02
+    eng.close()
+
+
+@pytest.mark.slow
+def test_mixed_precision_matches_fp64_statistically():
+    mix, conds, _ = ekmc.workload("cfg2_n2like_scan", n_electrons=20000, conditions=[2, 6])
+    res = {}
+    for prec, seed in ((ekmc.FP64, 21), (ekmc.MIXED, 22)):
+        num = _short_numerics(seed, prec)
+        num.n_integration_points = 1500
+        eng = _engine(mix, conds, num)
+        eng.solve()
+        res[prec] = [eng.results(c) for c in range(2)]
+        eng.close()
+    for a, b in zip(res[ekmc.FP64], res[ekmc.MIXED]):
+        assert _combined_sigma(a.mean_energy, a.mean_energy_error, b.mean_energy, b.mean_energy_error)
+        assert _combined_sigma(a.flux_drift_velocity[2], a.flux_drift_velocity_error[2],
+                               b.flux_drift_velocity[2], b.flux_drift_velocity_error[2])
