@@ -429,6 +429,10 @@ __device__ inline void collide(const CondParams& c, const DevProcesses& pr, cons
     o.pid = chosen;
     const int type = pr.type[chosen];
     const V inc = e.eps;
+    if (type == 2) {
+        o.dE = -(double)inc;  // attachmentCollision: the electron is lost
+        return;
+    }
     if (type == 0) {
         // conservativeCollision
         double M = pr.tMass[chosen], mu = pr.rMass[chosen], loss = pr.energyLoss[chosen];
@@ -511,8 +515,6 @@ __device__ inline void collide(const CondParams& c, const DevProcesses& pr, cons
         o.ejEps = ej;
         e.eps = sc_e;
         o.dE = -thr;
-    } else {
-        o.dE = -(double)inc;  // attachment
     }
 }
 

@@ -69,6 +69,10 @@ def test_collisions_match_oracle_with_identical_variates(wl, ioniz, gastemp):
     pid_g, v_g, e_g, ejv_g, eje_g, de_g = eng.op_collide(0, maxE, v, e, nue, var)
     st, pid_o, v_o, e_o, ejv_o, eje_o, de_o = m.collide(v, e, nue, var)
     assert st == 0
+    bad = np.nonzero((pid_g != pid_o) | ~np.isclose(de_g, de_o, rtol=1e-11, atol=1e-12))[0]
+    for i in bad[:5]:
+        print("mismatch", i, "pid", pid_g[i], pid_o[i], "e", repr(e[i]), "de", de_g[i], de_o[i],
+              "eout", e_g[i], e_o[i], "v", v_g.reshape(-1, 3)[i], v_o.reshape(-1, 3)[i], "var", var[i])
     assert np.array_equal(pid_g, pid_o)
     assert (pid_g >= 0).sum() > n // 20
     np.testing.assert_allclose(v_g, v_o, rtol=1e-12, atol=1e-12 * np.abs(v_o).max())
@@ -76,6 +80,49 @@ def test_collisions_match_oracle_with_identical_variates(wl, ioniz, gastemp):
     np.testing.assert_allclose(ejv_g, ejv_o, rtol=1e-12, atol=1e-12 * max(np.abs(ejv_o).max(), 1.0))
     np.testing.assert_allclose(eje_g, eje_o, rtol=1e-12)
     np.testing.assert_allclose(de_g, de_o, rtol=1e-11, atol=1e-12)
+    eng.close()
+
+
+@pytest.mark.parametrize("wl", ["cfg3_o2like_batched", "cfg5_mix5_large"])
+def test_every_channel_forced_matches_oracle(wl):
+    """Each process of the mixture forced in turn (the uniform for the channel
+    choice placed inside that process's cumulative band): same index, same
+    energy change for conservative, ionization and attachment processes."""
+    mix, conds, num = ekmc.workload(wl, n_electrons=1024, conditions=[0])
+    num.n_interp_points = 4000
+    num.gas_temperature_effect = 0
+    eng = _engine(mix, conds, num)
+    m = O.Model(mix, conds[0], num)
+    maxE = 90.0
+    interp, cumul, tot, mx, step = m.tables(maxE)
+    rows = []
+    for p in range(mix.n_processes):
+        for e0 in (5.0, 6.5, 7.28, 13.0, 30.0, 61.0):
+            k = int(e0 / step)
+            w1 = (k + 1) - e0 / step
+            c = w1 * cumul[k] + (1 - w1) * cumul[k + 1]
+            lo = c[p - 1] if p > 0 else 0.0
+            if c[p] > lo:
+                rows.append((p, e0, 0.5 * (lo + c[p])))
+    n = len(rows)
+    rng = np.random.default_rng(13)
+    e = np.array([r[1] for r in rows])
+    v = unit_vectors(rng, n) * speeds(e)[:, None]
+    e = 0.5 * ekmc.EMASS * (v * v).sum(1) / ekmc.CHARGE
+    nue = np.full(n, mx[-1] * 1.05)
+    var = rng.uniform(0.01, 0.99, (n, ekmc.VARIATES_PER_EVENT))
+    var[:, 5] = np.array([r[2] for r in rows]) * conds[0].gas_density * np.linalg.norm(v, axis=1) / nue
+    pg, vg, eg, ejvg, ejeg, deg = eng.op_collide(0, maxE, v, e, nue, var)
+    st, po, vo, eo, ejvo, ejeo, deo = m.collide(v, e, nue, var)
+    want = np.array([r[0] for r in rows])
+    types = mix.arrays["type"][want]
+    assert set(types.tolist()) >= {0, 1} and (wl != "cfg3_o2like_batched" or 2 in types)
+    assert np.array_equal(po, want) or np.all((po == want) | (po == -1))
+    assert np.array_equal(pg, po)
+    np.testing.assert_allclose(deg, deo, rtol=1e-11, atol=1e-12)
+    np.testing.assert_allclose(vg, vo, rtol=1e-12, atol=1e-12 * np.abs(vo).max())
+    att = types == 2
+    np.testing.assert_allclose(deg[att & (pg >= 0)], -e[att & (pg >= 0)], rtol=1e-15)
     eng.close()
 
 
