@@ -271,6 +271,7 @@ typedef struct {
     double steady_state_time;     /* NON_DEF (-123456789) before the steady state */
     double last_mean_energy;      /* [eV] */
     double max_elec_energy;       /* [eV] */
+    double table_max_energy;      /* upper end of the current collision tables [eV] */
     uint64_t real_collisions, null_collisions;
     int32_t n_sampling_points, n_integration_points;
     int32_t finished;

@@ -1252,6 +1252,7 @@ ekmc_status ekmc_get_status(const ekmc_engine* e, int32_t cond, ekmc_cond_status
     out->steady_state_time = C.ssTime;
     out->last_mean_energy = C.nSamp > 0 ? C.sE[C.nSamp - 1] : 0;
     out->max_elec_energy = C.maxElecEnergy;
+    out->table_max_energy = C.tab.maxEnergy;
     out->real_collisions = C.total;
     out->null_collisions = C.nullc;
     out->n_sampling_points = C.nSamp;

@@ -119,7 +119,8 @@ class Results(C.Structure):
 class CondStatus(C.Structure):
     _fields_ = [("time", C.c_double), ("trial_collision_frequency", C.c_double),
                 ("steady_state_time", C.c_double), ("last_mean_energy", C.c_double),
-                ("max_elec_energy", C.c_double), ("real_collisions", C.c_uint64),
+                ("max_elec_energy", C.c_double), ("table_max_energy", C.c_double),
+                ("real_collisions", C.c_uint64),
                 ("null_collisions", C.c_uint64), ("n_sampling_points", C.c_int32),
                 ("n_integration_points", C.c_int32), ("finished", C.c_int32), ("pad", C.c_int32)]
 

@@ -189,6 +189,7 @@ def test_initial_ensemble_and_one_interval_match_oracle():
     trial = st.trial_collision_frequency
     assert np.all(t1 == tsync)
     m = O.Model(mix, cond, num)
+    m.tables(st.table_max_energy)
     n = cond.n_electrons
     state = dict(pos=pos0, vel=vel0, energy=e0, time=t0, free_time=np.full(n, ekmc.NON_DEF, float),
                  nu_e=np.full(n, trial), stream=np.arange(n, dtype=np.uint64),
@@ -227,11 +228,14 @@ def test_runs_are_bitwise_reproducible():
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("wl,idx", [("cfg3_o2like_batched", [21]), ("cfg3_o2like_batched", [3])])
+@pytest.mark.parametrize("wl,idx", [("cfg3_o2like_batched", [21]), ("cfg3_o2like_batched", [12])])
 def test_population_control_keeps_n_and_balances_power(wl, idx):
-    """21: 1000 Td, ionization-dominated growth; 3: 2.7 Td, attachment-dominated."""
+    """21: 1000 Td, ionization-dominated growth; 12: 52 Td, attachment-dominated
+    (net loss of electrons refilled by clones).  In steady state the power
+    balance of BoltzmannMC.C:1769-1785 closes once the growth term is included."""
     mix, conds, _ = ekmc.workload(wl, n_electrons=20000, conditions=idx)
     num = _short_numerics(5)
+    num.max_collisions_before_ss = 400
     eng = _engine(mix, conds, num)
     eng.solve()
     r = eng.results(0)
@@ -239,6 +243,8 @@ def test_population_control_keeps_n_and_balances_power(wl, idx):
     assert np.all(np.isfinite(e)) and np.all(t == t[0])
     assert len(np.unique(np.round(vel[:, 0], 3))) > 0.9 * len(vel)  # no mass duplication
     rates, counts, gain, loss = eng.process_data(0)
+    print("rates ion/att", r.total_ion_rate_coeff, r.total_att_rate_coeff, "growth", r.power_growth,
+          "field", r.power_field, "balance", r.power_balance_rel_error)
     if idx == [21]:
         assert r.total_ion_rate_coeff > r.total_att_rate_coeff and r.power_growth < 0
     else:
