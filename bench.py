@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Collision events/s of the electron Monte Carlo engine on one MI355X.
+
+Workload: the 64-condition batched E/N x B/N scan of the synthetic O2-like
+gas (22 channels, dissociative attachment and ionization, crossed fields for
+B/N > 0, 1e4..1e6 electrons per condition, 15.6 M electrons in total).
+
+A step is one synchronization interval of every condition: free flights and
+collisions of all electrons to the interval end, device population control,
+ensemble moments, and the host control logic (trial frequency, tables,
+sampling).  `value` is the batched mixed-precision form (all 64 conditions in
+one set of launches); the plain FP64 one-condition-per-launch form runs the
+same conditions for comparison, and the oracle (a CPU port of the
+reference loop) is timed on a bounded sample on the host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "loki-mc_amd", "python"))
+
+import numpy as np  # noqa: E402
+
+import ekmc  # noqa: E402
+
+WORKLOAD = "cfg3_o2like_batched"
+METRIC = "collision events/s on 1 MI355X, batched mixed-prec vs plain FP64; swarm err"
+HBM_PEAK_GBS = 8000.0
+
+
+def state_bytes(precision):
+    """Bytes of one electron's state: x, y, z, t (FP64), vx, vy, vz, eps, free
+    time, trial frequency (FP64 or FP32), stream (8), event (4), alive (1)."""
+    v = 8 if precision == ekmc.FP64 else 4
+    return 4 * 8 + 6 * v + 8 + 4 + 1
+
+
+def run_engine(mix, conds, num, warmup, steps):
+    eng = ekmc.Engine(mix, conds, num)
+    eng.initialize()
+    if warmup:
+        eng.step(warmup)
+    eng.reset_kernel_stats()
+    ev0, real0 = eng.event_count()
+    t0 = time.perf_counter()
+    eng.step(steps)
+    t1 = time.perf_counter()
+    ev1, real1 = eng.event_count()
+    stats = eng.kernel_stats()
+    eng.close()
+    return ev1 - ev0, real1 - real0, t1 - t0, stats
+
+
+def plain_fp64(mix, conds, base_num, warmup, steps):
+    """One condition per engine (and per launch), FP64: the drop-in form."""
+    events = real = 0
+    seconds = 0.0
+    adv_ms = adv_launch = 0
+    for i, c in enumerate(conds):
+        num = ekmc.Numerics.from_buffer_copy(base_num)
+        num.precision = ekmc.FP64
+        num.seed = base_num.seed + i
+        ev, re, sec, st = run_engine(mix, [c], num, warmup, steps)
+        events += ev
+        real += re
+        seconds += sec
+        adv_launch += st["advance"][0]
+        adv_ms += st["advance"][1]
+    return events, real, seconds, adv_ms / max(adv_launch, 1)
+
+
+def cpu_baseline(mix, conds, num, intervals, cap):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(threads, 16))
+    events = 0
+    seconds = 0.0
+    for c in conds:
+        cc = ekmc.Condition.from_buffer_copy(c)
+        cc.n_electrons = min(int(c.n_electrons), cap)
+        m = oracle_py.Model(mix, cc, num)
+        ev, _, sec = m.run_intervals(intervals, threads)
+        events += ev
+        seconds += sec
+    return events / seconds, threads, seconds
+
+
+def calibrate_copy(mbytes, reps=4):
+    """Device-to-device copies of a known size (the runtime's blit kernel), so
+    that HBM counters of a profile can be scaled by what they read for a
+    plain copy of known bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = int(mbytes) << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": n, "reps": reps}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--plain-steps", type=int, default=100)
+    ap.add_argument("--plain-warmup", type=int, default=100)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-intervals", type=int, default=60)
+    ap.add_argument("--cpu-cap", type=int, default=20000)
+    ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB")
+    args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs one process on one MI355X; "
+              "--gpus %d is not supported (the conditions of the scan shard across GPUs "
+              "without communication, see DESIGN.md)." % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    mix, conds, num = ekmc.workload(WORKLOAD)
+    num.precision = ekmc.MIXED
+    num.seed = args.seed
+    n_total = int(sum(c.n_electrons for c in conds))
+    print("bench: %s, %d conditions, %d electrons, %d processes" %
+          (WORKLOAD, len(conds), n_total, mix.n_processes), file=sys.stderr, flush=True)
+
+    events, real, secs, stats = run_engine(mix, conds, num, args.warmup, args.steps)
+    value = events / secs
+    ms_per_step = 1e3 * secs / args.steps
+    print("bench: batched mixed %.3e events/s (%d events in %.3f s)" % (value, events, secs),
+          file=sys.stderr, flush=True)
+    adv_launches, adv_ms = stats["advance"]
+    adv_avg_ms = adv_ms / max(adv_launches, 1)
+    # algorithmic bytes of one flight launch: every electron's state read once
+    # and written once (table reads hit L2/MALL and are not counted)
+    adv_bytes = 2.0 * n_total * state_bytes(ekmc.MIXED)
+    achieved = adv_bytes / (adv_avg_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "advance_mixed_traffic.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 flight/scatter + fp64 clock/positions/reductions",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD + ": 64 conditions, E/N 1-1000 Td x B/N {0,200,1000} Hx, "
+                               "O2-like 22 channels, 1e4-1e6 electrons per condition",
+                   "conditions": len(conds), "electrons": n_total, "processes": mix.n_processes,
+                   "interval": "1 / trial collision frequency"},
+        "real_fraction": real / events,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_advance (flight+collision)", "avg_ms": adv_avg_ms,
+                     "bytes_per_launch": adv_bytes},
+        "kernels_ms_per_step": {k: v[1] / max(args.steps, 1) for k, v in stats.items() if v[0]},
+    }
+    if not args.no_plain:
+        pe, pr, ps, padv = plain_fp64(mix, conds, num, args.plain_warmup, args.plain_steps)
+        out["plain_fp64"] = {"value": pe / ps, "events": pe, "seconds": ps, "steps": args.plain_steps,
+                             "advance_avg_ms": padv}
+        out["speedup_vs_plain_fp64"] = value / (pe / ps)
+        print("bench: plain fp64 %.3e events/s" % (pe / ps), file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        v, threads, secs_cpu = cpu_baseline(mix, conds, num, args.cpu_intervals, args.cpu_cap)
+        out["cpu_baseline"] = {"value": v, "unit": "events/s", "cores": threads, "kind": "port",
+                               "sample": "oracle (CPU port of the reference loop) on all 64 conditions "
+                                         "with electrons capped at %d, %d intervals each, %.1f s"
+                                         % (args.cpu_cap, args.cpu_intervals, secs_cpu)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1320,6 +1320,31 @@ int orc_solve(orc_model* mp, int n_threads, ekmc_results* out, double* rates, do
     return EKMC_OK;
 }
 
+int orc_run_intervals(orc_model* mp, int n_threads, int n_sync, uint64_t* events, uint64_t* real,
+                      double* seconds) {
+    orc_model& m = *mp;
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#endif
+    evaluate_non_constant(m);
+    m.nSamp = 1; m.nSync = 1; m.idx = 0; m.sT[0] = 0;
+    calculate_mean(m);
+    auto start = std::chrono::high_resolution_clock::now();
+    for (int k = 0; k < n_sync; ++k) {
+        electron_dynamics_until_sync(m);
+        ++m.nSamp;
+        m.idx = m.nSamp - 1;
+        if ((int)m.sT.size() <= m.idx) m.sT.resize(2 * m.nSamp, 0.0);
+        m.sT[m.idx] = m.time;
+        calculate_mean(m);
+    }
+    auto end = std::chrono::high_resolution_clock::now();
+    *events = m.total + m.nullc;
+    *real = m.total;
+    *seconds = std::chrono::duration<double>(end - start).count();
+    return EKMC_OK;
+}
+
 int orc_series(const orc_model* m, int32_t* n_points, double* times, double* mean_energy,
                double* mean_velocity) {
     int n = std::min(*n_points, m->nSamp);

@@ -86,6 +86,12 @@ int64_t orc_advance_pool(const orc_model* m, uint64_t seed, double trial_freq, d
  * threads; fills the same results struct as the engine. */
 int orc_solve(orc_model* m, int n_threads, ekmc_results* out, double* rate_coeffs,
               double* eedf_cells, double* eedf);
+/* The reference loop for a fixed number of synchronization intervals (each
+ * with its passes, serial population control and ensemble sampling) from a
+ * fresh ensemble, timed around the interval loop only.  Used as the CPU
+ * baseline of the benchmark. */
+int orc_run_intervals(orc_model* m, int n_threads, int n_sync, uint64_t* events, uint64_t* real,
+                      double* seconds);
 /* Sampled series after orc_solve (same layout as ekmc_get_series). */
 int orc_series(const orc_model* m, int32_t* n_points, double* times, double* mean_energy,
                double* mean_velocity);

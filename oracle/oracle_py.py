@@ -53,6 +53,7 @@ def lib():
         L.orc_advance_pool.restype = I64
         L.orc_solve.argtypes = [C.c_void_p, C.c_int, _p(ekmc.Results), pD, pD, pD]
         L.orc_series.argtypes = [C.c_void_p, _p(I32), pD, pD, pD]
+        L.orc_run_intervals.argtypes = [C.c_void_p, C.c_int, C.c_int, _p(U64), _p(U64), pD]
         L.orc_statistical_error.argtypes = [pD, I64, D]
         L.orc_statistical_error.restype = D
         L.orc_histogram.argtypes = [pD, I64, pD, I32, pD]
@@ -156,6 +157,13 @@ class Model:
         if st:
             raise RuntimeError("orc_solve failed: %d" % st)
         return r, rates, cells, eedf
+
+    def run_intervals(self, n_sync, n_threads=1):
+        ev, real, sec = U64(), U64(), D()
+        st = lib().orc_run_intervals(self.h, n_threads, n_sync, C.byref(ev), C.byref(real), C.byref(sec))
+        if st:
+            raise RuntimeError("orc_run_intervals failed: %d" % st)
+        return ev.value, real.value, sec.value
 
     def series(self):
         n = C.c_int32(0)
