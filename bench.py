@@ -33,11 +33,17 @@ METRIC = "collision events/s on 1 MI355X, batched mixed-prec vs plain FP64; swar
 HBM_PEAK_GBS = 8000.0
 
 
-def state_bytes(precision):
-    """Bytes of one electron's state: x, y, z, t (FP64), vx, vy, vz, eps, free
-    time, trial frequency (FP64 or FP32), stream (8), event (4), alive (1)."""
-    v = 8 if precision == ekmc.FP64 else 4
-    return 4 * 8 + 6 * v + 8 + 4 + 1
+def record_bytes(precision):
+    """Bytes of one electron record in HBM: x, y, z, t (FP64); vx, vy, vz, free
+    time, trial frequency (FP32 mixed / FP64 plain); event (4), stream (8);
+    padded to 64 (mixed) or 96 (FP64) bytes."""
+    return 96 if precision == ekmc.FP64 else 64
+
+
+def interval_bytes(precision):
+    """Algorithmic HBM bytes of one electron in one flight launch: its record
+    read and written once, its alive flag read."""
+    return 2 * record_bytes(precision) + 1
 
 
 def run_engine(mix, conds, num, warmup, steps):
@@ -48,9 +54,12 @@ def run_engine(mix, conds, num, warmup, steps):
     eng.reset_kernel_stats()
     ev0, real0 = eng.event_count()
     t0 = time.perf_counter()
-    eng.step(steps)
+    running = eng.step(steps)
     t1 = time.perf_counter()
     ev1, real1 = eng.event_count()
+    if running != len(conds):
+        raise RuntimeError("%d of %d conditions stopped inside the timed region"
+                           % (len(conds) - running, len(conds)))
     stats = eng.kernel_stats()
     eng.close()
     return ev1 - ev0, real1 - real0, t1 - t0, stats
@@ -136,6 +145,9 @@ def main():
     mix, conds, num = ekmc.workload(WORKLOAD)
     num.precision = ekmc.MIXED
     num.seed = args.seed
+    # every condition keeps running through warmup and the timed steps, so a
+    # step is always one interval of the whole 64-condition batch
+    num.n_integration_points = 1e15
     n_total = int(sum(c.n_electrons for c in conds))
     print("bench: %s, %d conditions, %d electrons, %d processes" %
           (WORKLOAD, len(conds), n_total, mix.n_processes), file=sys.stderr, flush=True)
@@ -147,9 +159,9 @@ def main():
           file=sys.stderr, flush=True)
     adv_launches, adv_ms = stats["advance"]
     adv_avg_ms = adv_ms / max(adv_launches, 1)
-    # algorithmic bytes of one flight launch: every electron's state read once
-    # and written once (table reads hit L2/MALL and are not counted)
-    adv_bytes = 2.0 * n_total * state_bytes(ekmc.MIXED)
+    # algorithmic bytes of one interval's flight launches: every electron's
+    # record read once and written once (table reads hit L2/MALL, not counted)
+    adv_bytes = float(n_total * interval_bytes(ekmc.MIXED))
     achieved = adv_bytes / (adv_avg_ms * 1e-3) / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "advance_mixed_traffic.json")
@@ -173,6 +185,7 @@ def main():
                    "conditions": len(conds), "electrons": n_total, "processes": mix.n_processes,
                    "interval": "1 / trial collision frequency"},
         "real_fraction": real / events,
+        "events_per_electron_interval": events / (n_total * args.steps),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_advance (flight+collision)", "avg_ms": adv_avg_ms,

@@ -14,6 +14,7 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr int kBlock = 256;            // threads per block of every grid-wide kernel
 constexpr int kWave = 64;              // CDNA wavefront
 constexpr int kMaxSkip = 4096;         // capacity of the removal-correction set per condition
+constexpr int kChunkDefault = 1024;    // main electrons per wave work unit (64 lanes x 16)
 constexpr uint32_t kMaxLoopPerSync = 1u << 22;  // flight cap per electron per interval
 
 // Field cases of accelerateElectron (BoltzmannMC.C:809-897)
@@ -24,7 +25,7 @@ enum Purpose : uint32_t { P_EVENT = 0, P_INIT = 1, P_POPCTL = 2, P_CHILD = 3, P_
 
 // Status bits raised by kernels (per condition)
 enum StatusBits : uint32_t {
-    ST_REGION_OVERFLOW = 1u,   // a thread produced more newborns than its region holds
+    ST_REGION_OVERFLOW = 1u,   // a chunk produced more newborns than its region holds
     ST_ZERO_VELOCITY = 2u,     // cart2sph of a null vector
     ST_LOOP_CAP = 4u,          // flight cap reached inside one interval
     ST_POPCTL = 8u,            // removal correction set too large
@@ -71,11 +72,8 @@ struct CondParams {
     int64_t base;            // first main slot
     int64_t nMain;           // N_c
     int64_t regionBase;      // first newborn-region slot
-    int32_t regionPerThread; // q_c
-    int32_t threads;         // T_c = blocks * kBlock
-    int32_t blockBegin;      // first block of this condition in the work grids
-    int32_t blocks;
-    int64_t threadBegin;     // global index of this condition's first thread
+    int32_t chunkBegin;      // first chunk of this condition
+    int32_t nChunks;         // chunks of this condition
     int32_t active;          // 0: condition finished, kernels skip it
     int32_t pad;
 };
@@ -93,9 +91,10 @@ struct PopPlan {
     uint32_t pad;
 };
 
-// Per-block partial record of the flight kernel: see DevLayout below.
-// Layout in doubles: [0,P) counts, [P,2P) gains, [2P,3P) losses, then
-// fieldGain, nNull, nReal, nEvents, nEscape, maxEnergy, electronTime.
+// Per-condition event record of one interval (summary of the flight and
+// population-control kernels), in doubles: [0,P) counts, [P,2P) gains,
+// [2P,3P) losses, then fieldGain, nNull, nReal, nEvents, nEscape, maxEnergy,
+// growth, pad.
 inline int event_record_size(int P) { return 3 * P + 8; }
 enum EventTail { EV_FIELD = 0, EV_NULL = 1, EV_REAL = 2, EV_EVENTS = 3, EV_ESCAPE = 4, EV_MAXE = 5,
                  EV_GROWTH = 6, EV_PAD = 7 };

@@ -107,6 +107,7 @@ __device__ __forceinline__ double energy_of(double vx, double vy, double vz) {
     return 0.5 * kMass * sqr_norm(vx, vy, vz) / kCharge;
 }
 __device__ __forceinline__ float energy_of(float vx, float vy, float vz) {
+#pragma clang fp contract(off)
     return (float)(0.5 * kMass / kCharge) * (vx * vx + vy * vy + vz * vz);
 }
 // FP64 energy of an FP32 velocity (threshold subtraction in the mixed form)

@@ -63,6 +63,17 @@ std::vector<double> linspaced(int n, double a, double b) {
     return out;
 }
 
+// Electron energy [eV] of a stored velocity, in the arithmetic of the
+// device's energy_of (the engine stores velocities, not energies).
+inline double host_energy(double vx, double vy, double vz) {
+#pragma clang fp contract(off)
+    return 0.5 * kMass * (vx * vx + vy * vy + vz * vz) / kCharge;
+}
+inline double host_energy(float vx, float vy, float vz) {
+#pragma clang fp contract(off)
+    return (double)((float)(0.5 * kMass / kCharge) * (vx * vx + vy * vy + vz * vz));
+}
+
 // Per-condition host state: the loop variables of BoltzmannMC.
 struct Cond {
     ekmc_condition in;
@@ -177,21 +188,25 @@ template <typename V> class Engine : public EngineBase {
 public:
     hipStream_t stream = nullptr;
     int I = 10000, P = 0, nCond = 0;
-    int64_t capacity = 0, totalMain = 0, nThreads = 0;
-    int nBlocks = 0;
+    int64_t capacity = 0, totalMain = 0;
+    int nChunks = 0, cus = 256;
     std::vector<CondParams> hcp;
+    std::vector<Chunk> hchunks;
     // device buffers
-    DevBuf<double> x, y, z, t;
-    DevBuf<V> vx, vy, vz, eps, tfree, nue;
-    DevBuf<uint64_t> strm;
-    DevBuf<uint32_t> evt, holeList, skipSet, status;
+    DevBuf<Rec<V>> rec;
+    DevBuf<uint32_t> holeList, skipSet, status, queue;
     DevBuf<uint8_t> alive;
     DevBuf<V> tables;
     DevBuf<CondParams> dconds;
-    DevBuf<int32_t> blockCond, holesT, surplusT, markT, slotCond, phaseRow, cylFlag;
+    DevBuf<Chunk> chunks;
+    DevBuf<int32_t> holesC, surplusC, markC, flist, phaseRow, cylFlag;
     DevBuf<int64_t> holeOff, surOff, markOff, mainBase;
-    DevBuf<double> evPart, momPart, summary, thermalDev, grid;
+    DevBuf<double> tailC, momC, summary, thermalDev, grid;
+    DevBuf<unsigned long long> accCount;
+    DevBuf<long long> accGain, accLoss;
     DevBuf<PopPlan> plans;
+    FlightLists fl = {};
+    std::vector<int32_t> listActive;  // active flags the flight lists were built for
     DevBuf<int> iproc;
     DevBuf<double> dproc;
     DevBuf<unsigned long long> hEedf, hPer, hEadf, hEvdf;
@@ -212,14 +227,13 @@ public:
             if (timer.start[k]) (void)hipEventDestroy(timer.start[k]);
             if (timer.stop[k]) (void)hipEventDestroy(timer.stop[k]);
         }
-        x.release(); y.release(); z.release(); t.release(); vx.release(); vy.release(); vz.release();
-        eps.release(); tfree.release(); nue.release(); strm.release(); evt.release(); holeList.release();
-        skipSet.release(); status.release(); alive.release(); tables.release(); dconds.release();
-        blockCond.release(); holesT.release(); surplusT.release(); markT.release(); slotCond.release();
-        phaseRow.release(); cylFlag.release(); holeOff.release(); surOff.release(); markOff.release();
-        mainBase.release(); evPart.release(); momPart.release(); summary.release(); thermalDev.release();
-        grid.release(); plans.release(); iproc.release(); dproc.release(); hEedf.release(); hPer.release();
-        hEadf.release(); hEvdf.release();
+        rec.release(); holeList.release(); skipSet.release(); status.release(); queue.release();
+        alive.release(); tables.release(); dconds.release(); chunks.release(); holesC.release();
+        surplusC.release(); markC.release(); flist.release(); phaseRow.release(); cylFlag.release();
+        holeOff.release(); surOff.release(); markOff.release(); mainBase.release(); tailC.release();
+        momC.release(); summary.release(); thermalDev.release(); grid.release(); accCount.release();
+        accGain.release(); accLoss.release(); plans.release(); iproc.release(); dproc.release();
+        hEedf.release(); hPer.release(); hEadf.release(); hEvdf.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -283,24 +297,27 @@ public:
         }
         nPhases = anyAC ? num.n_integration_phases : 0;
         if (anyAC && nPhases < 1) return fail(EKMC_ERR_INVALID_ARGUMENT, "nIntegrationPhases must be >= 1");
-        // ---- work layout: blocks per condition proportional to N_c
-        int devId = 0, cus = 256;
+        // ---- work layout: chunks of ~work_chunk main electrons, one wavefront each
+        int devId = 0;
         (void)hipGetDevice(&devId);
         hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, devId) == hipSuccess && prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;
-        const int64_t target = (int64_t)cus * 8;
+        size_t ldsMax = 65536;
+        if (hipGetDeviceProperties(&prop, devId) == hipSuccess) {
+            if (prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;
+            if (prop.sharedMemPerBlock > 0) ldsMax = prop.sharedMemPerBlock;
+        }
+        if ((size_t)(kBlock / kWave) * 3 * P * 8 > ldsMax)
+            return fail(EKMC_ERR_INVALID_ARGUMENT, "too many processes for the per-wave collision tallies in LDS");
+        if (num.work_chunk < 0 || num.reserved != 0) return fail(EKMC_ERR_INVALID_ARGUMENT, "invalid work_chunk/reserved");
+        int64_t chunkTarget = num.work_chunk > 0 ? num.work_chunk : kChunkDefault;
+        chunkTarget = std::max<int64_t>(kWave, (chunkTarget + kWave - 1) / kWave * kWave);
         hcp.assign(n, CondParams());
+        hchunks.clear();
         int64_t base = 0;
-        nBlocks = 0;
-        nThreads = 0;
-        std::vector<int32_t> hBlockCond;
         for (int c = 0; c < n; ++c) {
             const int64_t Nc = conds[c].n_electrons;
-            int64_t maxBlocks = (Nc + kBlock - 1) / kBlock;
-            int64_t bl = std::max<int64_t>(1, std::min<int64_t>(maxBlocks, (int64_t)std::llround((double)target * Nc / (double)sumN)));
-            int64_t T = bl * kBlock;
-            int64_t perThread = (Nc + T - 1) / T;
-            int64_t q = std::max<int64_t>(8, 2 * perThread);
+            const int64_t nch = (Nc + chunkTarget - 1) / chunkTarget;
+            if ((int64_t)hchunks.size() + nch > 0x7fffffffLL) return fail(EKMC_ERR_CAPACITY, "too many work chunks");
             CondParams& cp = hcp[c];
             std::memset(&cp, 0, sizeof(cp));
             for (int k = 0; k < 3; ++k) { cp.E[k] = cs[c].E[k]; cp.accel[k] = cs[c].accel[k]; }
@@ -315,39 +332,56 @@ public:
             cp.base = base;
             cp.nMain = Nc;
             cp.regionBase = base + Nc;
-            cp.regionPerThread = (int32_t)q;
-            cp.threads = (int32_t)T;
-            cp.blockBegin = nBlocks;
-            cp.blocks = (int32_t)bl;
-            cp.threadBegin = nThreads;
+            cp.chunkBegin = (int32_t)hchunks.size();
+            cp.nChunks = (int32_t)nch;
             cp.active = 1;
-            base += Nc + q * T;
-            for (int64_t b = 0; b < bl; ++b) hBlockCond.push_back(c);
-            nBlocks += (int)bl;
-            nThreads += T;
+            // balanced chunk sizes; each chunk's newborn region holds twice its electrons
+            int64_t first = base, region = base + Nc;
+            for (int64_t j = 0; j < nch; ++j) {
+                Chunk ch;
+                ch.cond = c;
+                ch.count = (int32_t)(Nc / nch + (j < Nc % nch ? 1 : 0));
+                ch.first = first;
+                ch.region = region;
+                ch.regionCap = (int32_t)std::max<int64_t>(256, (2 * (int64_t)ch.count + kWave - 1) / kWave * kWave);
+                ch.local = (int32_t)j;
+                first += ch.count;
+                region += ch.regionCap;
+                hchunks.push_back(ch);
+            }
+            base = region;
         }
+        nChunks = (int)hchunks.size();
         capacity = base;
         totalMain = sumN;
         if (capacity > 0xFFFFFFF0LL) return fail(EKMC_ERR_CAPACITY, "electron pool exceeds 2^32 slots");
         // ---- device allocations
-        HIP_TRY(x.alloc(capacity)); HIP_TRY(y.alloc(capacity)); HIP_TRY(z.alloc(capacity)); HIP_TRY(t.alloc(capacity));
-        HIP_TRY(vx.alloc(capacity)); HIP_TRY(vy.alloc(capacity)); HIP_TRY(vz.alloc(capacity));
-        HIP_TRY(eps.alloc(capacity)); HIP_TRY(tfree.alloc(capacity)); HIP_TRY(nue.alloc(capacity));
-        HIP_TRY(strm.alloc(capacity)); HIP_TRY(evt.alloc(capacity)); HIP_TRY(alive.alloc(capacity));
+        HIP_TRY(rec.alloc(capacity));
+        HIP_TRY(alive.alloc(capacity));
         HIP_TRY(holeList.alloc(capacity));
         HIP_TRY(hipMemsetAsync(alive.p, 0, capacity, stream));
         HIP_TRY(tables.alloc((size_t)n * I * (2 * P + 1)));
         HIP_TRY(dconds.alloc(n));
-        HIP_TRY(blockCond.alloc(nBlocks));
-        HIP_TRY(hipMemcpyAsync(blockCond.p, hBlockCond.data(), nBlocks * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-        HIP_TRY(holesT.alloc(nThreads)); HIP_TRY(surplusT.alloc(nThreads)); HIP_TRY(markT.alloc(nThreads));
-        HIP_TRY(holeOff.alloc(nThreads)); HIP_TRY(surOff.alloc(nThreads)); HIP_TRY(markOff.alloc(nThreads));
-        HIP_TRY(hipMemsetAsync(markT.p, 0, nThreads * sizeof(int32_t), stream));
+        HIP_TRY(chunks.alloc(nChunks));
+        HIP_TRY(hipMemcpyAsync(chunks.p, hchunks.data(), nChunks * sizeof(Chunk), hipMemcpyHostToDevice, stream));
+        HIP_TRY(holesC.alloc(nChunks)); HIP_TRY(surplusC.alloc(nChunks)); HIP_TRY(markC.alloc(nChunks));
+        HIP_TRY(holeOff.alloc(nChunks)); HIP_TRY(surOff.alloc(nChunks)); HIP_TRY(markOff.alloc(nChunks));
+        HIP_TRY(flist.alloc(nChunks));
+        HIP_TRY(hipMemsetAsync(holesC.p, 0, nChunks * sizeof(int32_t), stream));
+        HIP_TRY(hipMemsetAsync(surplusC.p, 0, nChunks * sizeof(int32_t), stream));
+        HIP_TRY(hipMemsetAsync(markC.p, 0, nChunks * sizeof(int32_t), stream));
         recSize = event_record_size(P);
         summarySize = recSize + kMomentRecord + 8;
-        HIP_TRY(evPart.alloc((size_t)nBlocks * recSize));
-        HIP_TRY(momPart.alloc((size_t)nBlocks * kMomentRecord));
+        HIP_TRY(tailC.alloc((size_t)nChunks * CT_SIZE));
+        HIP_TRY(momC.alloc((size_t)nChunks * kMomentRecord));
+        HIP_TRY(hipMemsetAsync(tailC.p, 0, (size_t)nChunks * CT_SIZE * sizeof(double), stream));
         HIP_TRY(summary.alloc((size_t)n * summarySize));
+        HIP_TRY(accCount.alloc((size_t)n * P)); HIP_TRY(accGain.alloc((size_t)n * P)); HIP_TRY(accLoss.alloc((size_t)n * P));
+        HIP_TRY(hipMemsetAsync(accCount.p, 0, (size_t)n * P * 8, stream));
+        HIP_TRY(hipMemsetAsync(accGain.p, 0, (size_t)n * P * 8, stream));
+        HIP_TRY(hipMemsetAsync(accLoss.p, 0, (size_t)n * P * 8, stream));
+        HIP_TRY(queue.alloc(8));
+        HIP_TRY(hipMemsetAsync(queue.p, 0, 8 * sizeof(uint32_t), stream));
         HIP_TRY(plans.alloc(n));
         HIP_TRY(skipSet.alloc((size_t)n * kMaxSkip));
         HIP_TRY(status.alloc(n));
@@ -390,30 +424,27 @@ public:
         std::vector<int32_t> cyl(n);
         for (int c = 0; c < n; ++c) cyl[c] = cs[c].cylSym ? 1 : 0;
         HIP_TRY(hipMemcpyAsync(cylFlag.p, cyl.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-        ds = DevState<V>{x.p, y.p, z.p, t.p, vx.p, vy.p, vz.p, eps.p, tfree.p, nue.p, strm.p, evt.p, alive.p};
-        dw.conds = dconds.p; dw.blockCond = blockCond.p; dw.evPart = evPart.p; dw.momPart = momPart.p;
-        dw.holesT = holesT.p; dw.surplusT = surplusT.p; dw.markT = markT.p; dw.holeOff = holeOff.p;
-        dw.surOff = surOff.p; dw.markOff = markOff.p; dw.holeList = holeList.p; dw.plans = plans.p;
-        dw.skipSet = skipSet.p; dw.status = status.p; dw.summary = summary.p; dw.summarySize = summarySize;
-        dw.recSize = recSize; dw.nBlocks = nBlocks; dw.nCond = n;
+        ds = DevState<V>{rec.p, alive.p};
+        dw.conds = dconds.p; dw.chunks = chunks.p; dw.holesC = holesC.p; dw.surplusC = surplusC.p;
+        dw.markC = markC.p; dw.holeOff = holeOff.p; dw.surOff = surOff.p; dw.markOff = markOff.p;
+        dw.holeList = holeList.p; dw.tailC = tailC.p; dw.momC = momC.p; dw.accCount = accCount.p;
+        dw.accGain = accGain.p; dw.accLoss = accLoss.p; dw.plans = plans.p; dw.skipSet = skipSet.p;
+        dw.status = status.p; dw.queue = queue.p; dw.summary = summary.p; dw.summarySize = summarySize;
+        dw.recSize = recSize; dw.nChunks = nChunks; dw.nCond = n;
         dh.eedf = hEedf.p; dh.eedfPer = hPer.p; dh.eadf = hEadf.p; dh.evdf = hEvdf.p; dh.grid = grid.p;
         dh.phaseRow = phaseRow.p; dh.cyl = cylFlag.p;
         dh.nCells = nCells; dh.nCos = nCos; dh.nRad = nRad; dh.nAx = nAx; dh.nPhases = nPhases;
-        // per-slot condition map for the init kernel
-        std::vector<int32_t> sc(totalMain);
+        // first main electron of each condition, for the init kernel
         std::vector<int64_t> mb(n);
         std::vector<double> td(n);
         int64_t off = 0;
         for (int c = 0; c < n; ++c) {
             mb[c] = off;
-            for (int64_t i = 0; i < conds[c].n_electrons; ++i) sc[off + i] = c;
             off += conds[c].n_electrons;
             td[c] = std::sqrt(kBoltzmann * num.initial_elec_temp_over_gas_temp * cs[c].Tg / kMass);
         }
-        HIP_TRY(slotCond.alloc(totalMain));
         HIP_TRY(mainBase.alloc(n));
         HIP_TRY(thermalDev.alloc(n));
-        HIP_TRY(hipMemcpyAsync(slotCond.p, sc.data(), totalMain * sizeof(int32_t), hipMemcpyHostToDevice, stream));
         HIP_TRY(hipMemcpyAsync(mainBase.p, mb.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, stream));
         HIP_TRY(hipMemcpyAsync(thermalDev.p, td.data(), n * sizeof(double), hipMemcpyHostToDevice, stream));
         hsum.assign((size_t)n * summarySize, 0.0);
@@ -474,8 +505,39 @@ public:
         return EKMC_OK;
     }
 
+    // Flight-kernel work lists of the running conditions, per field case.
+    std::vector<int32_t> hlist;
+    ekmc_status rebuild_lists() {
+        std::vector<int32_t> act(nCond);
+        for (int c = 0; c < nCond; ++c) act[c] = cs[c].finished ? 0 : 1;
+        if (act == listActive) return EKMC_OK;
+        listActive = act;
+        hlist.clear();
+        fl = FlightLists{};
+        const bool thermal = num.gas_temperature_effect != EKMC_GASTEMP_FALSE;
+        for (int fc = 0; fc < 5; ++fc) {
+            const size_t begin = hlist.size();
+            for (int c = 0; c < nCond; ++c) {
+                if (!act[c] || cs[c].fieldCase != fc) continue;
+                for (int j = 0; j < hcp[c].nChunks; ++j) hlist.push_back(hcp[c].chunkBegin + j);
+            }
+            const int32_t count = (int32_t)(hlist.size() - begin);
+            fl.list[fc] = flist.p + begin;
+            fl.count[fc] = count;
+            if (count) {
+                const int64_t waves = (int64_t)flight_blocks_per_cu<V>(fc, thermal, P) * cus;
+                const int64_t need = (count + kBlock / kWave - 1) / (kBlock / kWave);
+                fl.grid[fc] = (int32_t)std::max<int64_t>(1, std::min<int64_t>(need, waves));
+            }
+        }
+        if (!hlist.empty())
+            HIP_TRY(hipMemcpyAsync(flist.p, hlist.data(), hlist.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return EKMC_OK;
+    }
+
     ekmc_status run_reduce_and_fetch() {
-        HIP_TRY(launch_reduce(dw, P, stream));
+        HIP_TRY(launch_finish(dw, P, stream));
         HIP_TRY(hipMemcpyAsync(hsum.data(), summary.p, hsum.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         sync_timers();
@@ -696,7 +758,7 @@ public:
         }
         HIP_TRY(hipMemcpyAsync(dconds.p, hcp.data(), nCond * sizeof(CondParams), hipMemcpyHostToDevice, stream));
         set_kernel_timer(&timer);
-        HIP_TRY(launch_init<V>(ds, dw, totalMain, mainBase.p, thermalDev.p, slotCond.p, stream));
+        HIP_TRY(launch_init<V>(ds, dw, totalMain, mainBase.p, thermalDev.p, stream));
         HIP_TRY(launch_sample<V>(ds, dw, stream));
         ekmc_status st = run_reduce_and_fetch();
         if (st != EKMC_OK) return st;
@@ -725,7 +787,6 @@ public:
         set_kernel_timer(&timer);
         for (int s = 0; s < nsync; ++s) {
             int active = 0;
-            uint32_t caseMask = 0;
             bool anyHist = false;
             std::vector<double> hgrid((size_t)nCond * 6, 0.0);
             std::vector<int32_t> hrow(nCond, -1);
@@ -737,7 +798,6 @@ public:
                 if (C.histRequest) { gr[5] = 1; anyHist = true; hrow[c] = C.histPhase; C.histRequest = 0; }
                 if (C.finished) continue;
                 ++active;
-                caseMask |= 1u << C.fieldCase;
                 check_max_collision_frequency(c);
                 if (C.gridDirty) {
                     ekmc_status st = upload_tables(c);
@@ -764,8 +824,12 @@ public:
                 HIP_TRY(launch_hist<V>(ds, dw, dh, stream));
             }
             if (active == 0) { HIP_TRY(hipStreamSynchronize(stream)); break; }
+            {
+                ekmc_status st = rebuild_lists();
+                if (st != EKMC_OK) return st;
+            }
             HIP_TRY(hipMemcpyAsync(dconds.p, hcp.data(), nCond * sizeof(CondParams), hipMemcpyHostToDevice, stream));
-            HIP_TRY(launch_interval<V>(ds, dw, dpr, tables.p, caseMask, stream));
+            HIP_TRY(launch_interval<V>(ds, dw, dpr, tables.p, fl, stream));
             HIP_TRY(launch_sample<V>(ds, dw, stream));
             ekmc_status st = run_reduce_and_fetch();
             if (st != EKMC_OK) return st;
@@ -895,22 +959,16 @@ public:
     ekmc_status get_electrons(int c, double* pos, double* vel, double* e, double* tt) override {
         const CondParams& cp = hcp[c];
         const int64_t n = cp.nMain;
-        std::vector<double> a(n), b(n), d(n);
-        std::vector<V> va(n), vb(n), vc(n), ve(n);
+        std::vector<Rec<V>> h(n);
         HIP_TRY(hipStreamSynchronize(stream));
-        HIP_TRY(hipMemcpy(a.data(), x.p + cp.base, n * 8, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(b.data(), y.p + cp.base, n * 8, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(d.data(), z.p + cp.base, n * 8, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(va.data(), vx.p + cp.base, n * sizeof(V), hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(vb.data(), vy.p + cp.base, n * sizeof(V), hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(vc.data(), vz.p + cp.base, n * sizeof(V), hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(ve.data(), eps.p + cp.base, n * sizeof(V), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(h.data(), rec.p + cp.base, n * sizeof(Rec<V>), hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < n; ++i) {
-            if (pos) { pos[3 * i] = a[i]; pos[3 * i + 1] = b[i]; pos[3 * i + 2] = d[i]; }
-            if (vel) { vel[3 * i] = va[i]; vel[3 * i + 1] = vb[i]; vel[3 * i + 2] = vc[i]; }
-            if (e) e[i] = ve[i];
+            const Rec<V>& q = h[i];
+            if (pos) { pos[3 * i] = q.x; pos[3 * i + 1] = q.y; pos[3 * i + 2] = q.z; }
+            if (vel) { vel[3 * i] = q.vx; vel[3 * i + 1] = q.vy; vel[3 * i + 2] = q.vz; }
+            if (e) e[i] = host_energy(q.vx, q.vy, q.vz);
+            if (tt) tt[i] = q.t;
         }
-        if (tt) HIP_TRY(hipMemcpy(tt, t.p + cp.base, n * 8, hipMemcpyDeviceToHost));
         return EKMC_OK;
     }
 
@@ -1082,6 +1140,8 @@ void ekmc_numerics_default(ekmc_numerics* n) {
     n->sync_over_sampling = 1;
     n->max_sync_points = 0;
     n->seed = 0;
+    n->work_chunk = 0;
+    n->reserved = 0;
 }
 
 ekmc_status ekmc_create(const ekmc_mixture* mixture, const ekmc_condition* conditions, int32_t n_cond,

@@ -1,4 +1,5 @@
-// kernels.h -- host-callable launchers of the gfx950 kernels (kernels.hip).
+// kernels.h -- device data layout and host-callable launchers of the gfx950
+// kernels (kernels.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -6,35 +7,72 @@
 
 namespace ekmc {
 
-// SoA electron pool.  V is double (plain form) or float (mixed form).
-template <typename V> struct DevState {
-    double *x, *y, *z, *t;
-    V *vx, *vy, *vz, *eps, *tfree, *nue;
-    uint64_t* stream;
-    uint32_t* event;
-    uint8_t* alive;
+// One electron in HBM, one record per pool slot (array of records, so that a
+// lane taking a new electron reads one contiguous 64- or 96-byte record).
+// The energy is not stored: it is 0.5 m_e |v|^2 / e of the stored velocity,
+// which is what the last flight of every electron computed.
+template <typename V> struct Rec;
+template <> struct __attribute__((aligned(16))) Rec<float> {
+    double x, y, z, t;          // position [m], clock [s]
+    float vx, vy, vz, tfree;    // velocity [m/s], pending free time [s] (< 0: none)
+    float nue;                  // trial frequency its free time was drawn with
+    uint32_t event;             // collision counter (Philox counter word)
+    uint64_t stream;            // Philox stream of this electron
 };
+template <> struct __attribute__((aligned(16))) Rec<double> {
+    double x, y, z, t;
+    double vx, vy, vz, tfree;
+    double nue;
+    uint32_t event, pad0;
+    uint64_t stream, pad1;
+};
+static_assert(sizeof(Rec<float>) == 64, "mixed record is 64 bytes");
+static_assert(sizeof(Rec<double>) == 96, "FP64 record is 96 bytes");
+
+template <typename V> struct DevState {
+    Rec<V>* rec;      // [pool capacity]
+    uint8_t* alive;   // [pool capacity] 0 dead/empty, 1 alive, 2 clone made this interval
+};
+
+// A wave's unit of work: up to `count` main electrons of one condition plus
+// the newborn region their ionizations fill.  The chunks of a condition are
+// contiguous in the chunk array and in the pool.
+struct Chunk {
+    int32_t cond;
+    int32_t count;        // main electrons [first, first + count)
+    int64_t first;        // first main slot; the chunk's hole list lives at holeList[first ..)
+    int64_t region;       // first newborn slot
+    int32_t regionCap;    // newborn capacity
+    int32_t local;        // index of the chunk within its condition
+};
+
+// Per-chunk scalar tallies (FP64, reduced in chunk order).
+enum ChunkTail { CT_FIELD = 0, CT_NULL, CT_REAL, CT_EVENTS, CT_ESCAPE, CT_MAXE, CT_GROWTH, CT_PAD, CT_SIZE };
 
 // Device work buffers shared by the interval kernels.
 struct DevWork {
     const CondParams* conds;      // [n_cond]
-    const int32_t* blockCond;     // [n_blocks] condition of each work block
-    double* evPart;               // [n_blocks * event_record_size(P)]
-    double* momPart;              // [n_blocks * kMomentRecord]
-    int32_t* holesT;              // [n_threads]
-    int32_t* surplusT;            // [n_threads]
-    int32_t* markT;               // [n_threads]
-    int64_t* holeOff;             // [n_threads] exclusive offsets within the condition
-    int64_t* surOff;              // [n_threads]
-    int64_t* markOff;             // [n_threads]
-    uint32_t* holeList;           // [pool capacity], strided per thread over main slots
+    const Chunk* chunks;          // [n_chunks]
+    int32_t* holesC;              // [n_chunks] holes left after the chunk's own refill
+    int32_t* surplusC;            // [n_chunks] newborns left in the chunk's region
+    int32_t* markC;               // [n_chunks]
+    int64_t* holeOff;             // [n_chunks] exclusive offsets within the condition
+    int64_t* surOff;
+    int64_t* markOff;
+    uint32_t* holeList;           // [pool capacity]; a chunk's holes at holeList[first ..)
+    double* tailC;                // [n_chunks * CT_SIZE]
+    double* momC;                 // [n_chunks * kMomentRecord]
+    unsigned long long* accCount; // [n_cond * P] real collisions per process
+    long long* accGain;           // [n_cond * P] fixed-point energy gains (2^-24 eV)
+    long long* accLoss;           // [n_cond * P]
     PopPlan* plans;               // [n_cond]
     uint32_t* skipSet;            // [n_cond * kMaxSkip]
     uint32_t* status;             // [n_cond]
-    double* summary;              // [n_cond * summary_size]
+    uint32_t* queue;              // [8] work-queue counters of the flight kernels
+    double* summary;              // [n_cond * summarySize]
     int summarySize;
-    int recSize;
-    int nBlocks;
+    int recSize;                  // 3P + 8
+    int nChunks;
     int nCond;
 };
 
@@ -52,18 +90,28 @@ struct DevHist {
     int nCells, nCos, nRad, nAx, nPhases;
 };
 
+// Flight-kernel work lists, one per field case: chunk ids in processing order.
+struct FlightLists {
+    const int32_t* list[5];
+    int32_t count[5];
+    int32_t grid[5];              // persistent blocks per launch
+};
+
+// mainBase[c]: index of condition c's first electron among all main electrons
 template <typename V>
 hipError_t launch_init(const DevState<V>& s, const DevWork& w, int64_t totalMain,
-                       const int64_t* mainBase, const double* thermalDev, const int32_t* slotCond,
-                       hipStream_t st);
+                       const int64_t* mainBase, const double* thermalDev, hipStream_t st);
 template <typename V>
 hipError_t launch_interval(const DevState<V>& s, const DevWork& w, const DevProcesses& pr,
-                           const V* tables, uint32_t caseMask, hipStream_t st);
+                           const V* tables, const FlightLists& fl, hipStream_t st);
 template <typename V>
 hipError_t launch_sample(const DevState<V>& s, const DevWork& w, hipStream_t st);
 template <typename V>
 hipError_t launch_hist(const DevState<V>& s, const DevWork& w, const DevHist& h, hipStream_t st);
-hipError_t launch_reduce(const DevWork& w, int P, hipStream_t st);
+hipError_t launch_finish(const DevWork& w, int P, hipStream_t st);
+// Resident blocks per CU of the flight kernel of field case fc.
+template <typename V>
+int flight_blocks_per_cu(int fc, bool thermal, int P);
 
 template <typename V>
 hipError_t launch_op_accelerate(const CondParams* cond, int64_t n, const double* time, const double* dt,
@@ -79,7 +127,7 @@ hipError_t launch_op_philox(int64_t n, const uint32_t* ctr, const uint32_t* key,
 enum KernelId { K_INIT = 0, K_ADVANCE, K_PLAN, K_MARKS, K_APPLY, K_PAIRS, K_MOVE, K_CLONE, K_SAMPLE,
                 K_HIST, K_REDUCE, K_COUNT };
 extern const char* kKernelNames[K_COUNT];
-// Per-kernel HIP events recorded by launch_interval/launch_sample when set.
+// Per-kernel HIP events recorded by the launchers when set.
 struct KernelTimer {
     hipEvent_t start[K_COUNT], stop[K_COUNT];
     bool armed[K_COUNT];

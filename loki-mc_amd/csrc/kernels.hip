@@ -1,35 +1,39 @@
-// kernels.hip -- gfx950 kernels of the electron Monte Carlo engine.
+// kernels.hip -- gfx950 kernels of the electron Monte Carlo path.
 //
-// One synchronization interval (BoltzmannMC.C:617-688 plus the population
-// control of :1282-1408) runs as this kernel sequence over all active
-// conditions at once:
-//   k_advance      each thread owns a fixed set of electrons of one condition
-//                  (main slots tl, tl+T, ...) and a private newborn region; it
-//                  flies every owned electron, and every electron ejected on
-//                  the way, to t_sync with the electron held in registers,
-//                  counts collisions into LDS, and refills its own holes
-//                  (attached electrons) from its own newborns.
-//   k_plan         per condition: scans of per-thread holes/surplus, the
-//                  population M, the number R of removals or C of clones.
-//   k_count_marks / k_plan_marks / k_apply_marks
-//                  exact uniform removal of R electrons: Bernoulli marks keyed
-//                  by electron identity, then a Floyd sample of marks to drop.
+// Work unit: a chunk of up to ~1k main electrons of one condition, owned by
+// one wavefront, with its own newborn region and hole list.  One
+// synchronization interval of every running condition is this sequence:
+//   k_advance      persistent per field case: each wave takes chunks from a
+//                  work queue and flies every electron of the chunk, and every
+//                  electron ejected on the way, to t_sync.  Lanes take the
+//                  next electron as soon as theirs is done (ballot-ranked
+//                  queue), so a wave never waits for its slowest electron.
+//                  Collisions are counted in LDS per wave and added to the
+//                  condition's integer accumulators; attached electrons leave
+//                  holes that the chunk's own newborns refill.
+//   k_plan         per condition: scans of holes/surplus over its chunks, the
+//                  population M, removals R or clones C.
+//   k_marks / k_plan_marks / k_apply_marks
+//                  exact uniform removal of R electrons: Bernoulli marks drawn
+//                  by geometric skips, then a Floyd sample of marks to keep.
 //   k_plan_pairs / k_move / k_clone
-//                  move surplus newborns into holes; clone uniformly chosen
-//                  electrons into the holes that remain (rejection, as the
+//                  surplus newborns fill holes; clones of uniformly chosen
+//                  electrons fill the holes that remain (rejection, as the
 //                  reference does at BoltzmannMC.C:1368-1371).
-//   k_sample       moments and histograms of the N electrons at t_sync
-//                  (BoltzmannMC.C:1410-1482, 1492-1572).
-//   k_reduce       fixed-order reduction of the per-block partial records.
-// Every reduction is in a fixed order (wave butterflies, then waves, then
-// blocks in index order) or on integers, so runs are bitwise reproducible.
+//   k_sample       FP64 moments of the N electrons at t_sync, per chunk
+//                  (BoltzmannMC.C:1410-1482).
+//   k_finish       per condition: fixed-order sums of the chunk records, the
+//                  integer accumulators, the plan; resets the queues.
+// Every floating-point reduction runs in a fixed order (wave butterflies, then
+// chunks in index order), everything else is integer, so runs are bitwise
+// reproducible whatever the order in which waves take chunks.
 #include "kernels.h"
 #include "device_physics.h"
 
 namespace ekmc {
 
-const char* kKernelNames[K_COUNT] = {"init", "advance", "plan", "count_marks", "apply_marks",
-                                     "plan_pairs", "move", "clone", "sample", "hist", "reduce"};
+const char* kKernelNames[K_COUNT] = {"init", "advance", "plan", "marks", "apply_marks",
+                                     "plan_pairs", "move", "clone", "sample", "hist", "finish"};
 
 static KernelTimer* g_timer = nullptr;
 void set_kernel_timer(KernelTimer* t) { g_timer = t; }
@@ -41,27 +45,15 @@ static inline void timer_stop(int k, hipStream_t st) {
     if (g_timer && g_timer->enabled) { (void)hipEventRecord(g_timer->stop[k], st); g_timer->armed[k] = true; }
 }
 
-constexpr double kFix = 16777216.0;  // 2^24: fixed-point scale of per-block energy sums [eV]
+constexpr double kFix = 16777216.0;  // 2^24: fixed-point scale of energy sums [eV]
+constexpr int kWavesPerBlock = kBlock / kWave;
 
-// ------------------------------------------------------------ state helpers
-template <typename V> __device__ __forceinline__ Electron<V> load_e(const DevState<V>& s, int64_t i) {
-    Electron<V> e;
-    e.x = s.x[i]; e.y = s.y[i]; e.z = s.z[i]; e.t = s.t[i];
-    e.vx = s.vx[i]; e.vy = s.vy[i]; e.vz = s.vz[i]; e.eps = s.eps[i];
-    e.tfree = s.tfree[i]; e.nue = s.nue[i]; e.stream = s.stream[i]; e.event = s.event[i];
-    return e;
+// ------------------------------------------------------------ wave helpers
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+// number of set bits of `mask` below this lane
+__device__ __forceinline__ int lane_rank(uint64_t mask) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
-template <typename V> __device__ __forceinline__ void store_e(const DevState<V>& s, int64_t i, const Electron<V>& e) {
-    s.x[i] = e.x; s.y[i] = e.y; s.z[i] = e.z; s.t[i] = e.t;
-    s.vx[i] = e.vx; s.vy[i] = e.vy; s.vz[i] = e.vz; s.eps[i] = e.eps;
-    s.tfree[i] = e.tfree; s.nue[i] = e.nue; s.stream[i] = e.stream; s.event[i] = e.event;
-}
-template <typename V> __device__ __forceinline__ void copy_e(const DevState<V>& s, int64_t src, int64_t dst) {
-    store_e(s, dst, load_e(s, src));
-    s.alive[dst] = 1;
-}
-
-// ------------------------------------------------------- block reductions
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
@@ -72,28 +64,75 @@ __device__ __forceinline__ double wave_max(double v) {
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
     return v;
 }
+__device__ __forceinline__ uint32_t wave_sum_u(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, kWave);
+    return v;
+}
+// exclusive prefix over the lanes of a wave
+__device__ __forceinline__ int wave_exclusive_scan(int v) {
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        int t = __shfl_up(incl, o, kWave);
+        if (lane_id() >= o) incl += t;
+    }
+    return incl - v;
+}
+// Wait until this wave's LDS operations are done (all lanes), so that other
+// lanes of the same wave see them.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+}
+
+// ------------------------------------------------------------ records
+template <typename V> __device__ __forceinline__ void load_rec(const Rec<V>* r, int64_t i, Electron<V>& e) {
+    const Rec<V> q = r[i];
+    e.x = q.x; e.y = q.y; e.z = q.z; e.t = q.t;
+    e.vx = q.vx; e.vy = q.vy; e.vz = q.vz;
+    e.eps = energy_of(q.vx, q.vy, q.vz);
+    e.tfree = q.tfree; e.nue = q.nue; e.stream = q.stream; e.event = q.event;
+}
+template <typename V> __device__ __forceinline__ void store_rec(Rec<V>* r, int64_t i, const Electron<V>& e) {
+    Rec<V> q = {};
+    q.x = e.x; q.y = e.y; q.z = e.z; q.t = e.t;
+    q.vx = e.vx; q.vy = e.vy; q.vz = e.vz; q.tfree = e.tfree;
+    q.nue = e.nue; q.event = e.event; q.stream = e.stream;
+    r[i] = q;
+}
+template <typename V> __device__ __forceinline__ double rec_energy(const Rec<V>& q) {
+    return (double)energy_of(q.vx, q.vy, q.vz);
+}
+
+// ------------------------------------------------------- block helpers
 // Sum over a kBlock-thread block in a fixed order; result valid in thread 0.
 __device__ double block_sum(double v, double* red) {
     v = wave_sum(v);
-    int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    int lane = lane_id(), wid = threadIdx.x / kWave;
     __syncthreads();
     if (lane == 0) red[wid] = v;
     __syncthreads();
     double r = 0;
     if (threadIdx.x == 0)
-        for (int i = 0; i < kBlock / kWave; ++i) r += red[i];
+        for (int i = 0; i < (int)(blockDim.x / kWave); ++i) r += red[i];
     __syncthreads();
     return r;
 }
 __device__ double block_max(double v, double* red) {
     v = wave_max(v);
-    int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    int lane = lane_id(), wid = threadIdx.x / kWave;
     __syncthreads();
     if (lane == 0) red[wid] = v;
     __syncthreads();
     double r = 0;
     if (threadIdx.x == 0)
-        for (int i = 0; i < kBlock / kWave; ++i) r = fmax(r, red[i]);
+        for (int i = 0; i < (int)(blockDim.x / kWave); ++i) r = fmax(r, red[i]);
     __syncthreads();
     return r;
 }
@@ -122,10 +161,10 @@ __device__ int64_t block_exclusive_scan(const int32_t* in, int64_t* out, int64_t
     return running;
 }
 
-// Position of a per-thread list element: the thread t whose exclusive offset
-// range [off[t], off[t+1]) contains i (largest t with off[t] <= i).
-__device__ __forceinline__ int64_t owner_of(const int64_t* off, int64_t T, int64_t i) {
-    int64_t lo = 0, hi = T - 1;
+// Index t of the range [off[t], off[t+1]) that contains i (largest t with
+// off[t] <= i) over n exclusive offsets.
+__device__ __forceinline__ int64_t owner_of(const int64_t* off, int64_t n, int64_t i) {
+    int64_t lo = 0, hi = n - 1;
     while (lo < hi) {
         int64_t mid = (lo + hi + 1) >> 1;
         if (off[mid] <= i) lo = mid; else hi = mid - 1;
@@ -144,12 +183,17 @@ __device__ __forceinline__ double u53(uint64_t seed, uint64_t stream, uint32_t e
 // Maxwellian at initialElecTempOverGasTemp * Tg (BoltzmannMC.C:491-516):
 // three normals from four uniforms (0,1] of Philox calls 0 and 1 of stream i.
 template <typename V>
-__global__ __launch_bounds__(kBlock) void k_init(DevState<V> s, const CondParams* conds, int64_t totalMain,
-                                                 const int64_t* mainBase, const double* thermalDev,
-                                                 const int32_t* slotCond) {
+__global__ __launch_bounds__(kBlock) void k_init(DevState<V> s, const CondParams* conds, int nCond,
+                                                 int64_t totalMain, const int64_t* mainBase,
+                                                 const double* thermalDev) {
     int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (g >= totalMain) return;
-    int c = slotCond[g];
+    int lo = 0, hi = nCond - 1;  // condition whose main range holds g
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (mainBase[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    const int c = lo;
     const CondParams& cp = conds[c];
     int64_t i = g - mainBase[c];
     int64_t slot = cp.base + i;
@@ -164,169 +208,242 @@ __global__ __launch_bounds__(kBlock) void k_init(DevState<V> s, const CondParams
     double aux1 = sqrt(-2.0 * log(r1)), aux2 = 2.0 * kPi * r2;
     double n0 = aux1 * cos(aux2), n1 = aux1 * sin(aux2), n2 = sqrt(-2.0 * log(r3)) * cos(2.0 * kPi * r4);
     double dev = thermalDev[c];
-    double vx = n0 * dev, vy = n1 * dev, vz = n2 * dev;
-    s.x[slot] = 0; s.y[slot] = 0; s.z[slot] = 0; s.t[slot] = 0;
-    s.vx[slot] = (V)vx; s.vy[slot] = (V)vy; s.vz[slot] = (V)vz;
-    s.eps[slot] = (V)energy_of(vx, vy, vz);
-    s.tfree[slot] = (V)kNonDef;
-    s.nue[slot] = (V)cp.trial;
-    s.stream[slot] = (uint64_t)i;
-    s.event[slot] = 0;
+    Electron<V> e;
+    e.x = 0; e.y = 0; e.z = 0; e.t = 0;
+    e.vx = (V)(n0 * dev); e.vy = (V)(n1 * dev); e.vz = (V)(n2 * dev);
+    e.eps = 0;
+    e.tfree = (V)kNonDef;
+    e.nue = (V)cp.trial;
+    e.stream = (uint64_t)i;
+    e.event = 0;
+    store_rec(s.rec, slot, e);
     s.alive[slot] = 1;
 }
 
-// --------------------------------------------------------------- advance
-struct AdvanceTally {
-    double fieldGain, maxE, nNull, nReal, nEvents, nEscape;
-    uint32_t status;
-};
+// ----------------------------------------------------- newborn refill
+// After a chunk's electrons are at t_sync (or after removals): surviving
+// newborns of the chunk's region, in region order, fill the chunk's holes in
+// hole-list order; the rest move to the front of the region.  Wave-level, in
+// groups of 64 newborns: every lane loads its record before any lane stores
+// (each store waits on the group's loads), sources are cleared before
+// destinations are set, and destinations never lie beyond the group.
+template <typename V>
+__device__ void wave_refill(const DevState<V>& s, uint32_t* holeList, const Chunk& ch, int& holes, int nNew,
+                            int& remaining) {
+    const int lane = lane_id();
+    int used = 0;
+    for (int b = 0; b < nNew; b += kWave) {
+        const int j = b + lane;
+        const bool al = j < nNew && s.alive[ch.region + j] != 0;
+        const uint64_t am = __ballot(al);
+        const int r = used + lane_rank(am);
+        int64_t dst = -1;
+        if (al) {
+            if (r < holes) dst = holeList[ch.first + r];
+            else if (r - holes != j) dst = ch.region + (r - holes);
+        }
+        if (dst >= 0) {
+            const Rec<V> q = s.rec[ch.region + j];
+            s.rec[dst] = q;
+            s.alive[ch.region + j] = 0;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (dst >= 0) s.alive[dst] = 1;
+        used += __popcll(am);
+    }
+    const int filled = used < holes ? used : holes;
+    for (int b = filled; b < holes; b += kWave) {  // remaining holes to the list front
+        const int h = b + lane;
+        uint32_t v = 0;
+        if (h < holes) v = holeList[ch.first + h];
+        __builtin_amdgcn_wave_barrier();
+        if (h < holes) holeList[ch.first + h - filled] = v;
+    }
+    holes -= filled;
+    remaining = used - filled;
+}
 
-// Fly one electron to t_sync (the passes of BoltzmannMC.C:630-681 for one
-// electron); ejected electrons are queued in the thread's newborn region.
-// Returns true when the electron was attached.
+// --------------------------------------------------------------- advance
+// One chunk: every main electron and every newborn flies to t_sync (the
+// passes of BoltzmannMC.C:630-681 for this set of electrons).
 template <typename V, int FC, bool THERMAL>
-__device__ __forceinline__ bool run_electron(const CondParams& cp, const DevProcesses& pr, const V* cumul,
-                                             const V* interp, const V* nutot, const DevState<V>& s,
-                                             Electron<V>& e, int64_t region, int64_t q, int& nq,
-                                             unsigned long long* ldsCount, long long* ldsGain,
-                                             long long* ldsLoss, AdvanceTally& tl) {
-    uint32_t iter = 0;
+__device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWork& w, const DevProcesses& pr,
+                                              const V* tables, int cid, unsigned long long* cnt) {
+    const int lane = lane_id();
+    const int P = pr.P;
+    long long* gainL = (long long*)(cnt + P);
+    long long* lossL = gainL + P;
+    const Chunk ch = w.chunks[cid];
+    const int c = ch.cond;
+    const CondParams cp = w.conds[c];
+    const int64_t I = cp.I;
+    const V* cumul = tables + cp.tableOffset;
+    const V* interp = cumul + I * P;
+    const V* nutot = interp + I * P;
+    for (int i = lane; i < 3 * P; i += kWave) ((long long*)cnt)[i] = 0;
+    wave_lds_sync();
+
+    const int count = ch.count;
+    int nextMain = 0, nq = 0, nextNew = 0, holes = 0;
+    Electron<V> e;
+    bool have = false, isNew = false, holeOnLoad = false;
+    int64_t slot = 0;
+    uint8_t al = 1;
+    uint32_t iter = 0, nNull = 0, nReal = 0, nEsc = 0, status = 0;
+    double fieldGain = 0, maxE = 0;
     while (true) {
+        // lanes without an electron take the next main electron, then newborns
+        const uint64_t need = __ballot(!have);
+        if (need) {
+            const int nNeed = __popcll(need);
+            const int availMain = count - nextMain;
+            const int takeMain = availMain < nNeed ? availMain : nNeed;
+            const int availNew = nq - nextNew;
+            const int takeNew = (nNeed - takeMain) < availNew ? (nNeed - takeMain) : availNew;
+            if (!have) {
+                const int r = lane_rank(need);
+                if (r < takeMain) {
+                    slot = ch.first + nextMain + r;
+                    isNew = false;
+                    have = true;
+                } else if (r - takeMain < takeNew) {
+                    slot = ch.region + nextNew + (r - takeMain);
+                    isNew = true;
+                    have = true;
+                }
+                if (have) {
+                    load_rec(s.rec, slot, e);
+                    al = isNew ? (uint8_t)1 : s.alive[slot];
+                    holeOnLoad = al == 0;
+                    iter = 0;
+                }
+            }
+            nextMain += takeMain;
+            nextNew += takeNew;
+        }
+        if (__ballot(have) == 0) break;
+
+        bool done = false, dead = false, spawn = false;
         CollOut<V> o;
-        double gain = 0;
-        int st = interval_step<V, FC, THERMAL>(cp, pr, cumul, interp, nutot, e, o, gain);
-        tl.fieldGain += gain;
-        if (st) return false;
-        if (!isfinite(e.t) || !isfinite((double)e.eps)) { tl.status |= ST_NONFINITE; e.t = cp.tsync; return false; }
-        tl.nEvents += 1;
-        if (o.escape) tl.nEscape += 1;
-        tl.status |= o.flags;
-        if (o.pid < 0) {
-            tl.nNull += 1;
-        } else {
-            tl.nReal += 1;
-            atomicAdd(&ldsCount[o.pid], 1ull);
-            long long fx = __double2ll_rn(o.dE * kFix);
-            if (o.dE >= 0) atomicAdd((unsigned long long*)&ldsGain[o.pid], (unsigned long long)fx);
-            else atomicAdd((unsigned long long*)&ldsLoss[o.pid], (unsigned long long)fx);
-            int type = pr.type[o.pid];
-            if (type == 1) {
-                if (nq < q) {
+        o.pid = -1;
+        if (have) {
+            if (holeOnLoad) {
+                done = true;
+                dead = true;
+            } else {
+                double gain = 0;
+                const int st = interval_step<V, FC, THERMAL>(cp, pr, cumul, interp, nutot, e, o, gain);
+                fieldGain += gain;
+                if (st) {
+                    done = true;
+                } else if (!isfinite(e.t) || !isfinite((double)e.eps)) {
+                    status |= ST_NONFINITE;
+                    e.t = cp.tsync;
+                    done = true;
+                } else {
+                    if (o.escape) ++nEsc;
+                    status |= o.flags;
+                    if (o.pid < 0) {
+                        ++nNull;
+                    } else {
+                        ++nReal;
+                        atomicAdd(&cnt[o.pid], 1ull);
+                        const long long fx = __double2ll_rn(o.dE * kFix);
+                        if (o.dE >= 0) atomicAdd((unsigned long long*)&gainL[o.pid], (unsigned long long)fx);
+                        else atomicAdd((unsigned long long*)&lossL[o.pid], (unsigned long long)fx);
+                        const int type = pr.type[o.pid];
+                        if (type == 1) spawn = true;
+                        else if (type == 2) { done = true; dead = true; }
+                    }
+                    if (++iter >= kMaxLoopPerSync) { status |= ST_LOOP_CAP; done = true; }
+                }
+            }
+        }
+        // ejected electrons join the chunk's newborn queue, in lane order
+        const uint64_t sp = __ballot(spawn);
+        if (sp) {
+            if (spawn) {
+                const int idx = nq + lane_rank(sp);
+                if (idx < ch.regionCap) {
                     Electron<V> n;
                     n.x = e.x; n.y = e.y; n.z = e.z; n.t = e.t;
                     n.vx = o.ejv[0]; n.vy = o.ejv[1]; n.vz = o.ejv[2]; n.eps = o.ejEps;
                     n.tfree = (V)kNonDef; n.nue = (V)cp.trial;
                     n.stream = child_stream(cp.seed, e.stream, e.event - 1);
                     n.event = 0;
-                    store_e(s, region + nq, n);
-                    s.alive[region + nq] = 1;
-                    ++nq;
+                    store_rec(s.rec, ch.region + idx, n);
+                    s.alive[ch.region + idx] = 1;
                 } else {
-                    tl.status |= ST_REGION_OVERFLOW;
+                    status |= ST_REGION_OVERFLOW;
                 }
-            } else if (type == 2) {
-                return true;
             }
+            const int tot = nq + __popcll(sp);
+            nq = tot < ch.regionCap ? tot : ch.regionCap;
         }
-        if (++iter >= kMaxLoopPerSync) { tl.status |= ST_LOOP_CAP; return false; }
+        // finished electrons: store, or leave a hole in the chunk's list
+        const uint64_t dm = __ballot(done && dead && !isNew);
+        if (done) {
+            if (dead) {
+                s.alive[slot] = 0;
+                if (!isNew) w.holeList[ch.first + holes + lane_rank(dm)] = (uint32_t)slot;
+            } else {
+                store_rec(s.rec, slot, e);
+                if (al != 1) s.alive[slot] = 1;
+                maxE = fmax(maxE, (double)e.eps);
+            }
+            have = false;
+        }
+        holes += __popcll(dm);
     }
+
+    int remaining = 0;
+    wave_refill<V>(s, w.holeList, ch, holes, nq, remaining);
+
+    const double fg = wave_sum(fieldGain);
+    const double mx = wave_max(maxE);
+    const uint32_t nn = wave_sum_u(nNull), nr = wave_sum_u(nReal), ne = wave_sum_u(nEsc);
+    const uint32_t stw = wave_or(status);
+    if (lane == 0) {
+        w.holesC[cid] = holes;
+        w.surplusC[cid] = remaining;
+        double* t = w.tailC + (size_t)cid * CT_SIZE;
+        t[CT_FIELD] = fg;
+        t[CT_NULL] = (double)nn;
+        t[CT_REAL] = (double)nr;
+        t[CT_EVENTS] = (double)nn + (double)nr;
+        t[CT_ESCAPE] = (double)ne;
+        t[CT_MAXE] = mx;
+        t[CT_GROWTH] = 0;
+        t[CT_PAD] = 0;
+        if (stw) atomicOr(&w.status[c], stw);
+    }
+    wave_lds_sync();
+    for (int p = lane; p < P; p += kWave) {
+        const unsigned long long n = cnt[p];
+        if (n) {
+            atomicAdd(&w.accCount[(size_t)c * P + p], n);
+            const long long gq = gainL[p], lq = lossL[p];
+            if (gq) atomicAdd((unsigned long long*)&w.accGain[(size_t)c * P + p], (unsigned long long)gq);
+            if (lq) atomicAdd((unsigned long long*)&w.accLoss[(size_t)c * P + p], (unsigned long long)lq);
+        }
+    }
+    wave_lds_sync();
 }
 
+// Persistent flight kernel of one field case: every wave takes chunks from
+// the list through an atomic counter until the list is exhausted.
 template <typename V, int FC, bool THERMAL>
-__global__ __launch_bounds__(kBlock) void k_advance(DevState<V> s, DevWork w, DevProcesses pr, const V* tables) {
+__global__ __launch_bounds__(kBlock) void k_advance(DevState<V> s, DevWork w, DevProcesses pr, const V* tables,
+                                                    const int32_t* list, int32_t nList, uint32_t* queue) {
     extern __shared__ unsigned char smem[];
-    const int b = blockIdx.x;
-    const int c = w.blockCond[b];
-    const CondParams cp = w.conds[c];
-    if (cp.fieldCase != FC) return;  // another instance flies this block
-    const int P = pr.P;
-    unsigned long long* ldsCount = (unsigned long long*)smem;      // [P]
-    long long* ldsGain = (long long*)(ldsCount + P);                // [P]
-    long long* ldsLoss = ldsGain + P;                               // [P]
-    double* red = (double*)(ldsLoss + P);                           // [kBlock/kWave]
-    double* part = w.evPart + (size_t)b * w.recSize;
-    for (int i = threadIdx.x; i < 3 * P; i += kBlock) ((long long*)smem)[i] = 0;
-    __syncthreads();
-    if (!cp.active) {
-        for (int i = threadIdx.x; i < w.recSize; i += kBlock) part[i] = 0;
-        return;
-    }
-    const int64_t T = cp.threads;
-    const int64_t tl = (int64_t)(b - cp.blockBegin) * kBlock + threadIdx.x;
-    const int64_t gt = cp.threadBegin + tl;
-    const int64_t I = cp.I;
-    const V* cumul = tables + cp.tableOffset;
-    const V* interp = cumul + I * P;
-    const V* nutot = interp + I * P;
-    const int64_t q = cp.regionPerThread;
-    const int64_t region = cp.regionBase + tl * q;
-
-    AdvanceTally ta = {0, 0, 0, 0, 0, 0, 0};
-    int nq = 0, holes = 0;
-    for (int64_t li = tl; li < cp.nMain; li += T) {
-        int64_t slot = cp.base + li;
-        if (!s.alive[slot]) { w.holeList[cp.base + tl + (int64_t)holes * T] = (uint32_t)slot; ++holes; continue; }
-        Electron<V> e = load_e(s, slot);
-        bool dead = run_electron<V, FC, THERMAL>(cp, pr, cumul, interp, nutot, s, e, region, q, nq, ldsCount,
-                                                 ldsGain, ldsLoss, ta);
-        if (dead) {
-            s.alive[slot] = 0;
-            w.holeList[cp.base + tl + (int64_t)holes * T] = (uint32_t)slot;
-            ++holes;
-        } else {
-            store_e(s, slot, e);
-            s.alive[slot] = 1;
-            ta.maxE = fmax(ta.maxE, (double)e.eps);
-        }
-    }
-    for (int j = 0; j < nq; ++j) {  // own newborns, including theirs
-        Electron<V> e = load_e(s, region + j);
-        bool dead = run_electron<V, FC, THERMAL>(cp, pr, cumul, interp, nutot, s, e, region, q, nq, ldsCount,
-                                                 ldsGain, ldsLoss, ta);
-        if (dead) s.alive[region + j] = 0;
-        else { store_e(s, region + j, e); ta.maxE = fmax(ta.maxE, (double)e.eps); }
-    }
-    // refill own holes from own newborns, compact the rest to the region front
-    int hi = 0, wr = 0;
-    for (int j = 0; j < nq; ++j) {
-        int64_t src = region + j;
-        if (!s.alive[src]) continue;
-        if (hi < holes) {
-            int64_t dst = w.holeList[cp.base + tl + (int64_t)hi * T];
-            copy_e(s, src, dst);
-            s.alive[src] = 0;
-            ++hi;
-        } else {
-            if (wr != j) { copy_e(s, src, region + wr); s.alive[src] = 0; }
-            ++wr;
-        }
-    }
-    for (int h = hi; h < holes; ++h)
-        w.holeList[cp.base + tl + (int64_t)(h - hi) * T] = w.holeList[cp.base + tl + (int64_t)h * T];
-    w.holesT[gt] = holes - hi;
-    w.surplusT[gt] = wr;
-    if (ta.status) atomicOr(&w.status[c], ta.status);
-
-    // block partial record: counts, gains, losses, then the scalar tail
-    double sums[5] = {ta.fieldGain, ta.nNull, ta.nReal, ta.nEvents, ta.nEscape};
-    double tail[5];
-    for (int k = 0; k < 5; ++k) tail[k] = block_sum(sums[k], red);
-    double mx = block_max(ta.maxE, red);
-    __syncthreads();
-    for (int p = threadIdx.x; p < P; p += kBlock) {
-        part[p] = (double)ldsCount[p];
-        part[P + p] = (double)ldsGain[p] / kFix;
-        part[2 * P + p] = (double)ldsLoss[p] / kFix;
-    }
-    if (threadIdx.x == 0) {
-        part[3 * P + EV_FIELD] = tail[0];
-        part[3 * P + EV_NULL] = tail[1];
-        part[3 * P + EV_REAL] = tail[2];
-        part[3 * P + EV_EVENTS] = tail[3];
-        part[3 * P + EV_ESCAPE] = tail[4];
-        part[3 * P + EV_MAXE] = mx;
-        part[3 * P + EV_GROWTH] = 0;
-        part[3 * P + EV_PAD] = 0;
+    const int wid = threadIdx.x / kWave;
+    unsigned long long* cnt = (unsigned long long*)smem + (size_t)wid * 3 * pr.P;
+    while (true) {
+        uint32_t item = 0;
+        if (lane_id() == 0) item = atomicAdd(queue, 1u);
+        item = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl((int)item, 0, kWave));
+        if (item >= (uint32_t)nList) break;
+        advance_chunk<V, FC, THERMAL>(s, w, pr, tables, list[item], cnt);
     }
 }
 
@@ -336,43 +453,74 @@ __global__ __launch_bounds__(1024) void k_plan(DevWork w) {
     const int c = blockIdx.x;
     const CondParams cp = w.conds[c];
     if (!cp.active) return;
-    int64_t H = block_exclusive_scan(w.holesT + cp.threadBegin, w.holeOff + cp.threadBegin, cp.threads, lds);
-    int64_t S = block_exclusive_scan(w.surplusT + cp.threadBegin, w.surOff + cp.threadBegin, cp.threads, lds);
+    const int64_t H = block_exclusive_scan(w.holesC + cp.chunkBegin, w.holeOff + cp.chunkBegin, cp.nChunks, lds);
+    const int64_t S = block_exclusive_scan(w.surplusC + cp.chunkBegin, w.surOff + cp.chunkBegin, cp.nChunks, lds);
     if (threadIdx.x == 0) {
         PopPlan pl = {};
         pl.holes = H;
         pl.surplus = S;
         pl.alive = cp.nMain - H + S;
         pl.remove = S > H ? S - H : 0;
-        double R = (double)pl.remove;
+        const double R = (double)pl.remove;
         pl.markProb = pl.remove > 0 ? fmin(1.0, (R + 8.0 * sqrt(R) + 20.0) / (double)pl.alive) : 0.0;
         w.plans[c] = pl;
     }
 }
 
-// Bernoulli mark of an alive electron (identity = its stream id)
-__device__ __forceinline__ bool is_marked(const CondParams& cp, double p, uint64_t stream) {
-    return u53(cp.seed, stream, (uint32_t)cp.syncIndex, P_POPCTL, (uint32_t)(cp.syncIndex >> 32) & 0xffffu) < p;
+// Bernoulli(p) marks over a chunk's candidates [0, L): main slots
+// [0, count) (holes are not alive and never count) and the surplus
+// newborns [count, L).  Lane l draws geometric skips over its segment
+// [l L / 64, (l+1) L / 64) with Philox keyed by (chunk, lane, interval).
+struct MarkGen {
+    uint64_t seed, key;
+    uint32_t sync, call;
+    double logq;
+    bool all;
+    int64_t pos, hi;
+    uint32_t w[4];
+    __device__ MarkGen(const CondParams& cp, const Chunk& ch, int lane, double p, int64_t L) {
+        seed = cp.seed;
+        key = ((uint64_t)(uint32_t)lane << 32) | (uint32_t)ch.local;
+        sync = (uint32_t)cp.syncIndex;
+        call = 0;
+        all = !(p < 1.0);
+        logq = all ? -1.0 : log1p(-p);
+        pos = (int64_t)lane * L / kWave - 1;
+        hi = (int64_t)(lane + 1) * L / kWave;
+    }
+    __device__ bool next() {
+        if (all) return ++pos < hi;
+        const bool h = (call & 1) != 0;
+        if (!h) philox_call(seed, key, sync, P_POPCTL, call >> 1, w);
+        ++call;
+        const uint32_t hiw = h ? w[2] : w[0], low = h ? w[3] : w[1];
+        const uint64_t x = ((((uint64_t)hiw) << 32) | low) >> 11;
+        const double u = ((double)x + 1.0) * (1.0 / 9007199254740992.0);  // (0,1]
+        const double skip = floor(log(u) / logq);
+        if (!(skip < (double)(hi - pos - 1))) { pos = hi; return false; }
+        pos += 1 + (int64_t)skip;
+        return true;
+    }
+};
+
+__device__ __forceinline__ bool mark_alive(const uint8_t* alive, const Chunk& ch, int64_t pos) {
+    return pos >= ch.count || alive[ch.first + pos] != 0;
 }
 
-template <typename V>
-__global__ __launch_bounds__(kBlock) void k_count_marks(DevState<V> s, DevWork w) {
-    const int b = blockIdx.x;
-    const int c = w.blockCond[b];
-    const CondParams cp = w.conds[c];
+__global__ __launch_bounds__(kBlock) void k_marks(const uint8_t* alive, DevWork w) {
+    const int lane = lane_id();
+    const int cid = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    if (cid >= w.nChunks) return;
+    const Chunk ch = w.chunks[cid];
+    const CondParams& cp = w.conds[ch.cond];
     if (!cp.active) return;
-    const int64_t T = cp.threads, tl = (int64_t)(b - cp.blockBegin) * kBlock + threadIdx.x, gt = cp.threadBegin + tl;
-    const PopPlan pl = w.plans[c];
-    if (pl.remove == 0) { w.markT[gt] = 0; return; }
-    int m = 0;
-    for (int64_t li = tl; li < cp.nMain; li += T) {
-        int64_t slot = cp.base + li;
-        if (s.alive[slot] && is_marked(cp, pl.markProb, s.stream[slot])) ++m;
-    }
-    int64_t region = cp.regionBase + tl * cp.regionPerThread;
-    for (int j = 0; j < w.surplusT[gt]; ++j)
-        if (is_marked(cp, pl.markProb, s.stream[region + j])) ++m;
-    w.markT[gt] = m;
+    const PopPlan pl = w.plans[ch.cond];
+    if (pl.remove == 0) return;
+    MarkGen g(cp, ch, lane, pl.markProb, (int64_t)ch.count + w.surplusC[cid]);
+    uint32_t m = 0;
+    while (g.next()) m += mark_alive(alive, ch, g.pos) ? 1u : 0u;
+    m = wave_sum_u(m);
+    if (lane == 0) w.markC[cid] = (int32_t)m;
 }
 
 // Scan of marks and Floyd's sample of the K - R marks that are kept alive.
@@ -385,7 +533,7 @@ __global__ __launch_bounds__(1024) void k_plan_marks(DevWork w) {
     if (!cp.active) return;
     PopPlan pl = w.plans[c];
     if (pl.remove == 0) return;
-    int64_t K = block_exclusive_scan(w.markT + cp.threadBegin, w.markOff + cp.threadBegin, cp.threads, lds);
+    int64_t K = block_exclusive_scan(w.markC + cp.chunkBegin, w.markOff + cp.chunkBegin, cp.nChunks, lds);
     int64_t m = K - pl.remove;
     bool bad = K < pl.remove || m > kMaxSkip;
     if (threadIdx.x == 0) {
@@ -403,7 +551,8 @@ __global__ __launch_bounds__(1024) void k_plan_marks(DevWork w) {
         int n = 0;
         const uint32_t mask = 2 * kMaxSkip - 1;
         for (int64_t j = K - m; j < K; ++j) {
-            double u = u53(cp.seed, (uint64_t)j, (uint32_t)cp.syncIndex, P_POPCTL, 0x8000u | ((uint32_t)(cp.syncIndex >> 32) & 0x7fffu));
+            double u = u53(cp.seed, (uint64_t)j, (uint32_t)cp.syncIndex, P_POPCTL,
+                           0x8000u | ((uint32_t)(cp.syncIndex >> 32) & 0x7fffu));
             uint32_t t = (uint32_t)fmin(floor(u * (double)(j + 1)), (double)j);
             uint32_t pick = t;
             for (int pass = 0; pass < 2; ++pass) {
@@ -450,62 +599,62 @@ __device__ __forceinline__ bool in_sorted(const uint32_t* a, int64_t n, uint32_t
     return lo < n && a[lo] == v;
 }
 
+// Marks in chunk order, then lane order, then position order; a mark whose
+// index is not in the kept set is removed.  Removed main electrons become
+// holes of the chunk, then the chunk refills them from its newborns.
 template <typename V>
 __global__ __launch_bounds__(kBlock) void k_apply_marks(DevState<V> s, DevWork w) {
-    __shared__ double red[kBlock / kWave];
-    const int b = blockIdx.x;
-    const int c = w.blockCond[b];
-    const CondParams cp = w.conds[c];
+    const int lane = lane_id();
+    const int cid = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    if (cid >= w.nChunks) return;
+    const Chunk ch = w.chunks[cid];
+    const CondParams& cp = w.conds[ch.cond];
     if (!cp.active) return;
-    const PopPlan pl = w.plans[c];
+    const PopPlan pl = w.plans[ch.cond];
     if (pl.remove == 0 || (pl.status & ST_POPCTL)) return;
-    const int64_t T = cp.threads, tl = (int64_t)(b - cp.blockBegin) * kBlock + threadIdx.x, gt = cp.threadBegin + tl;
-    const uint32_t* skip = w.skipSet + (size_t)c * kMaxSkip;
-    int64_t gi = w.markOff[gt];
+    const uint32_t* skip = w.skipSet + (size_t)ch.cond * kMaxSkip;
+    const int sur = w.surplusC[cid];
+    const int64_t L = (int64_t)ch.count + sur;
+    int mine = 0;
+    {
+        MarkGen g(cp, ch, lane, pl.markProb, L);
+        while (g.next()) mine += mark_alive(s.alive, ch, g.pos) ? 1 : 0;
+    }
+    int64_t gi = w.markOff[cid] + wave_exclusive_scan(mine);
+    int holes = w.holesC[cid];
     double growth = 0;
-    int holes = w.holesT[gt];
-    for (int64_t li = tl; li < cp.nMain; li += T) {
-        int64_t slot = cp.base + li;
-        if (!s.alive[slot] || !is_marked(cp, pl.markProb, s.stream[slot])) continue;
-        if (!in_sorted(skip, pl.skips, (uint32_t)gi)) {
+    MarkGen g(cp, ch, lane, pl.markProb, L);
+    bool more = g.next();
+    while (__ballot(more)) {
+        bool kill = false, killMain = false;
+        int64_t slot = 0;
+        if (more) {
+            if (mark_alive(s.alive, ch, g.pos)) {
+                if (!in_sorted(skip, pl.skips, (uint32_t)gi)) {
+                    kill = true;
+                    killMain = g.pos < ch.count;
+                    slot = killMain ? ch.first + g.pos : ch.region + (g.pos - ch.count);
+                }
+                ++gi;
+            }
+        }
+        const uint64_t km = __ballot(killMain);
+        if (kill) {
+            growth -= rec_energy(s.rec[slot]);
             s.alive[slot] = 0;
-            growth -= (double)s.eps[slot];
-            w.holeList[cp.base + tl + (int64_t)holes * T] = (uint32_t)slot;
-            ++holes;
+            if (killMain) w.holeList[ch.first + holes + lane_rank(km)] = (uint32_t)slot;
         }
-        ++gi;
+        holes += __popcll(km);
+        if (more) more = g.next();
     }
-    int64_t region = cp.regionBase + tl * cp.regionPerThread;
-    int sur = w.surplusT[gt];
-    for (int j = 0; j < sur; ++j) {
-        if (!is_marked(cp, pl.markProb, s.stream[region + j])) continue;
-        if (!in_sorted(skip, pl.skips, (uint32_t)gi)) {
-            s.alive[region + j] = 0;
-            growth -= (double)s.eps[region + j];
-        }
-        ++gi;
+    int remaining = 0;
+    wave_refill<V>(s, w.holeList, ch, holes, sur, remaining);
+    const double gsum = wave_sum(growth);
+    if (lane == 0) {
+        w.holesC[cid] = holes;
+        w.surplusC[cid] = remaining;
+        w.tailC[(size_t)cid * CT_SIZE + CT_GROWTH] += gsum;
     }
-    // refill own holes from own surviving newborns, compact the rest
-    int hi = 0, wr = 0;
-    for (int j = 0; j < sur; ++j) {
-        int64_t src = region + j;
-        if (!s.alive[src]) continue;
-        if (hi < holes) {
-            int64_t dst = w.holeList[cp.base + tl + (int64_t)hi * T];
-            copy_e(s, src, dst);
-            s.alive[src] = 0;
-            ++hi;
-        } else {
-            if (wr != j) { copy_e(s, src, region + wr); s.alive[src] = 0; }
-            ++wr;
-        }
-    }
-    for (int h = hi; h < holes; ++h)
-        w.holeList[cp.base + tl + (int64_t)(h - hi) * T] = w.holeList[cp.base + tl + (int64_t)h * T];
-    w.holesT[gt] = holes - hi;
-    w.surplusT[gt] = wr;
-    double g = block_sum(growth, red);
-    if (threadIdx.x == 0) w.evPart[(size_t)b * w.recSize + (w.recSize - 8) + EV_GROWTH] += g;
 }
 
 __global__ __launch_bounds__(1024) void k_plan_pairs(DevWork w) {
@@ -515,8 +664,8 @@ __global__ __launch_bounds__(1024) void k_plan_pairs(DevWork w) {
     if (!cp.active) return;
     PopPlan pl = w.plans[c];
     if (pl.holes == 0 && pl.surplus == 0) return;
-    int64_t H = block_exclusive_scan(w.holesT + cp.threadBegin, w.holeOff + cp.threadBegin, cp.threads, lds);
-    int64_t S = block_exclusive_scan(w.surplusT + cp.threadBegin, w.surOff + cp.threadBegin, cp.threads, lds);
+    const int64_t H = block_exclusive_scan(w.holesC + cp.chunkBegin, w.holeOff + cp.chunkBegin, cp.nChunks, lds);
+    const int64_t S = block_exclusive_scan(w.surplusC + cp.chunkBegin, w.surOff + cp.chunkBegin, cp.nChunks, lds);
     if (threadIdx.x == 0) {
         pl.holes = H;
         pl.surplus = S;
@@ -524,42 +673,53 @@ __global__ __launch_bounds__(1024) void k_plan_pairs(DevWork w) {
     }
 }
 
+// Pair i (i < min(H, S)): hole i of the condition <- surplus newborn i.  The
+// waves of a condition's chunks stride over the pairs.
 template <typename V>
 __global__ __launch_bounds__(kBlock) void k_move(DevState<V> s, DevWork w) {
-    const int b = blockIdx.x;
-    const int c = w.blockCond[b];
-    const CondParams cp = w.conds[c];
+    const int lane = lane_id();
+    const int cid = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    if (cid >= w.nChunks) return;
+    const Chunk me = w.chunks[cid];
+    const CondParams& cp = w.conds[me.cond];
     if (!cp.active) return;
-    const PopPlan pl = w.plans[c];
+    const PopPlan pl = w.plans[me.cond];
     const int64_t n = pl.holes < pl.surplus ? pl.holes : pl.surplus;
-    const int64_t T = cp.threads, tl = (int64_t)(b - cp.blockBegin) * kBlock + threadIdx.x;
-    const int64_t* hoff = w.holeOff + cp.threadBegin;
-    const int64_t* soff = w.surOff + cp.threadBegin;
-    for (int64_t i = tl; i < n; i += T) {
-        int64_t th = owner_of(hoff, T, i);
-        int64_t dst = w.holeList[cp.base + th + (i - hoff[th]) * T];
-        int64_t ts = owner_of(soff, T, i);
-        int64_t src = cp.regionBase + ts * cp.regionPerThread + (i - soff[ts]);
-        copy_e(s, src, dst);
+    const int64_t* hoff = w.holeOff + cp.chunkBegin;
+    const int64_t* soff = w.surOff + cp.chunkBegin;
+    const int64_t stride = (int64_t)cp.nChunks * kWave;
+    for (int64_t i = (int64_t)me.local * kWave + lane; i < n; i += stride) {
+        const int64_t th = owner_of(hoff, cp.nChunks, i);
+        const Chunk& hc = w.chunks[cp.chunkBegin + th];
+        const int64_t dst = w.holeList[hc.first + (i - hoff[th])];
+        const int64_t ts = owner_of(soff, cp.nChunks, i);
+        const Chunk& sc = w.chunks[cp.chunkBegin + ts];
+        const int64_t src = sc.region + (i - soff[ts]);
+        s.rec[dst] = s.rec[src];
         s.alive[src] = 0;
+        s.alive[dst] = 1;
     }
 }
 
+// Holes S..H-1 get clones of uniformly drawn alive main electrons.
 template <typename V>
 __global__ __launch_bounds__(kBlock) void k_clone(DevState<V> s, DevWork w) {
-    __shared__ double red[kBlock / kWave];
-    const int b = blockIdx.x;
-    const int c = w.blockCond[b];
-    const CondParams cp = w.conds[c];
+    const int lane = lane_id();
+    const int cid = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    if (cid >= w.nChunks) return;
+    const Chunk me = w.chunks[cid];
+    const CondParams& cp = w.conds[me.cond];
     if (!cp.active) return;
-    const PopPlan pl = w.plans[c];
-    const int64_t T = cp.threads, tl = (int64_t)(b - cp.blockBegin) * kBlock + threadIdx.x;
-    const int64_t* hoff = w.holeOff + cp.threadBegin;
+    const PopPlan pl = w.plans[me.cond];
+    if (pl.holes <= pl.surplus) return;
+    const int64_t* hoff = w.holeOff + cp.chunkBegin;
+    const int64_t stride = (int64_t)cp.nChunks * kWave;
     double growth = 0;
     uint32_t status = 0;
-    for (int64_t i = pl.surplus + tl; i < pl.holes; i += T) {
-        int64_t th = owner_of(hoff, T, i);
-        int64_t dst = w.holeList[cp.base + th + (i - hoff[th]) * T];
+    for (int64_t i = pl.surplus + (int64_t)me.local * kWave + lane; i < pl.holes; i += stride) {
+        const int64_t th = owner_of(hoff, cp.nChunks, i);
+        const Chunk& hc = w.chunks[cp.chunkBegin + th];
+        const int64_t dst = w.holeList[hc.first + (i - hoff[th])];
         // uniform alive source by rejection over the main slots (BoltzmannMC.C:1368-1371)
         int64_t src = -1;
         for (uint32_t k = 0; k < (1u << 20); ++k) {
@@ -568,43 +728,42 @@ __global__ __launch_bounds__(kBlock) void k_clone(DevState<V> s, DevWork w) {
             if (s.alive[cp.base + cand] == 1) { src = cp.base + cand; break; }
         }
         if (src < 0) { status |= ST_POPCTL; continue; }
-        Electron<V> e = load_e(s, src);
-        growth += (double)e.eps;
+        Rec<V> q = s.rec[src];
+        growth += rec_energy(q);
         uint32_t wds[4];
-        philox_call(cp.seed ^ 0x5851F42D4C957F2Dull, e.stream, (uint32_t)dst, P_CLONE,
+        philox_call(cp.seed ^ 0x5851F42D4C957F2Dull, q.stream, (uint32_t)dst, P_CLONE,
                     (uint32_t)(cp.syncIndex & 0xffffu), wds);
-        e.stream = ((uint64_t)wds[1] << 32) | wds[0];
-        e.event = 0;
-        store_e(s, dst, e);
+        q.stream = ((uint64_t)wds[1] << 32) | wds[0];
+        q.event = 0;
+        s.rec[dst] = q;
         s.alive[dst] = 2;  // alive, but not a clone source in this pass (BoltzmannMC.C:1374)
     }
-    if (status) atomicOr(&w.status[c], status);
-    double g = block_sum(growth, red);
-    if (threadIdx.x == 0) w.evPart[(size_t)b * w.recSize + (w.recSize - 8) + EV_GROWTH] += g;
+    const uint32_t st = wave_or(status);
+    const double g = wave_sum(growth);
+    if (lane == 0) {
+        if (st) atomicOr(&w.status[me.cond], st);
+        w.tailC[(size_t)cid * CT_SIZE + CT_GROWTH] += g;
+    }
 }
 
 // ----------------------------------------------------------------- sample
 // Ensemble sums of one sampling instant (calculateMeanDataForSwarmParams,
-// BoltzmannMC.C:1410-1482) in FP64: eps, r, v, r r^T, r v^T, count, max eps.
+// BoltzmannMC.C:1410-1482) in FP64, per chunk: eps, r, v, r r^T, r v^T,
+// count, max eps.
 template <typename V>
 __global__ __launch_bounds__(kBlock) void k_sample(DevState<V> s, DevWork w) {
-    __shared__ double red[kBlock / kWave];
-    const int b = blockIdx.x;
-    const int c = w.blockCond[b];
-    const CondParams cp = w.conds[c];
-    double* part = w.momPart + (size_t)b * kMomentRecord;
-    if (!cp.active) {
-        if (threadIdx.x < kMomentRecord) part[threadIdx.x] = 0;
-        return;
-    }
-    const int64_t T = cp.threads, tl = (int64_t)(b - cp.blockBegin) * kBlock + threadIdx.x;
+    const int lane = lane_id();
+    const int cid = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    if (cid >= w.nChunks) return;
+    const Chunk ch = w.chunks[cid];
+    if (!w.conds[ch.cond].active) return;
     double m[kMomentRecord];
     for (int k = 0; k < kMomentRecord; ++k) m[k] = 0;
-    for (int64_t li = tl; li < cp.nMain; li += T) {
-        int64_t slot = cp.base + li;
-        double r[3] = {s.x[slot], s.y[slot], s.z[slot]};
-        double v[3] = {(double)s.vx[slot], (double)s.vy[slot], (double)s.vz[slot]};
-        double e = (double)s.eps[slot];
+    for (int64_t k = lane; k < ch.count; k += kWave) {
+        const Rec<V> q = s.rec[ch.first + k];
+        const double r[3] = {q.x, q.y, q.z};
+        const double v[3] = {(double)q.vx, (double)q.vy, (double)q.vz};
+        const double e = rec_energy(q);
         m[0] += e;
         for (int a = 0; a < 3; ++a) { m[1 + a] += r[a]; m[4 + a] += v[a]; }
         for (int a = 0; a < 3; ++a)
@@ -612,52 +771,62 @@ __global__ __launch_bounds__(kBlock) void k_sample(DevState<V> s, DevWork w) {
         m[25] += 1;
         m[26] = fmax(m[26], e);
     }
-    for (int k = 0; k < kMomentRecord - 1; ++k) {
-        double t = block_sum(m[k], red);
-        if (threadIdx.x == 0) part[k] = t;
-    }
-    double mx = block_max(m[26], red);
-    if (threadIdx.x == 0) part[26] = mx;
+    for (int k = 0; k < kMomentRecord - 1; ++k) m[k] = wave_sum(m[k]);
+    m[26] = wave_max(m[26]);
+    if (lane == 0)
+        for (int k = 0; k < kMomentRecord; ++k) w.momC[(size_t)cid * kMomentRecord + k] = m[k];
 }
 
 // Histograms of one sampling instant (getTimeDependDistributions,
 // BoltzmannMC.C:1551-1571; MathFunctions::histogramCount/histogram2DCount,
-// MathFunctions.C:61-81,106-127), added to integer accumulators.
+// MathFunctions.C:61-81,106-127), added to integer accumulators.  The EEDF is
+// counted per wave in LDS when it fits, else directly in HBM.
 template <typename V>
-__global__ __launch_bounds__(kBlock) void k_hist(DevState<V> s, DevWork w, DevHist h) {
+__global__ __launch_bounds__(kBlock) void k_hist(DevState<V> s, DevWork w, DevHist h, int ldsCells) {
     extern __shared__ unsigned char smem[];
-    uint32_t* lhist = (uint32_t*)smem;  // [nCells]
-    const int b = blockIdx.x;
-    const int c = w.blockCond[b];
-    const CondParams cp = w.conds[c];
+    const int lane = lane_id(), wid = threadIdx.x / kWave;
+    const int cid = blockIdx.x * kWavesPerBlock + wid;
+    if (cid >= w.nChunks) return;
+    const Chunk ch = w.chunks[cid];
+    const int c = ch.cond;
     const double* g = h.grid + 6 * (size_t)c;
     if (g[5] == 0) return;
-    for (int i = threadIdx.x; i < h.nCells; i += kBlock) lhist[i] = 0;
-    __syncthreads();
-    const int64_t T = cp.threads, tl = (int64_t)(b - cp.blockBegin) * kBlock + threadIdx.x;
+    const bool useLds = ldsCells >= h.nCells;
+    uint32_t* lhist = (uint32_t*)smem + (size_t)wid * ldsCells;
+    if (useLds) {
+        for (int i = lane; i < h.nCells; i += kWave) lhist[i] = 0;
+        wave_lds_sync();
+    }
     const double eStep = g[0], cStep = g[1], rStep = g[2], aStep = g[3], maxSpeed = g[4];
     const bool cyl = h.cyl[c] != 0 && h.eadf;
-    for (int64_t li = tl; li < cp.nMain; li += T) {
-        int64_t slot = cp.base + li;
-        double e = (double)s.eps[slot];
-        int cell = (int)(e / eStep);
-        if (cell >= 0 && cell < h.nCells) atomicAdd(&lhist[cell], 1u);
+    const int row = h.phaseRow[c];
+    for (int64_t k = lane; k < ch.count; k += kWave) {
+        const Rec<V> q = s.rec[ch.first + k];
+        const double e = rec_energy(q);
+        const int cell = (int)(e / eStep);
+        if (cell >= 0 && cell < h.nCells) {
+            if (useLds) atomicAdd(&lhist[cell], 1u);
+            else {
+                atomicAdd(&h.eedf[(size_t)c * h.nCells + cell], 1ull);
+                if (row >= 0 && h.eedfPer) atomicAdd(&h.eedfPer[((size_t)c * h.nPhases + row) * h.nCells + cell], 1ull);
+            }
+        }
         if (cyl) {
-            double vx = s.vx[slot], vy = s.vy[slot], vz = s.vz[slot];
-            double cosA = vz / sqrt(vx * vx + vy * vy + vz * vz);
-            int cy = (int)((cosA + 1.0) / cStep);
+            const double vx = q.vx, vy = q.vy, vz = q.vz;
+            const double cosA = vz / sqrt(vx * vx + vy * vy + vz * vz);
+            const int cy = (int)((cosA + 1.0) / cStep);
             if (cell < h.nCells && cell >= 0 && cy < h.nCos && cy >= 0)
                 atomicAdd(&h.eadf[((size_t)c * h.nCells + cell) * h.nCos + cy], 1ull);
-            int rx = (int)(sqrt(vx * vx + vy * vy) / rStep);
-            int az = (int)((vz + maxSpeed) / aStep);
+            const int rx = (int)(sqrt(vx * vx + vy * vy) / rStep);
+            const int az = (int)((vz + maxSpeed) / aStep);
             if (rx < h.nRad && rx >= 0 && az < h.nAx && az >= 0)
                 atomicAdd(&h.evdf[((size_t)c * h.nRad + rx) * h.nAx + az], 1ull);
         }
     }
-    __syncthreads();
-    const int row = h.phaseRow[c];
-    for (int i = threadIdx.x; i < h.nCells; i += kBlock) {
-        uint32_t n = lhist[i];
+    if (!useLds) return;
+    wave_lds_sync();
+    for (int i = lane; i < h.nCells; i += kWave) {
+        const uint32_t n = lhist[i];
         if (!n) continue;
         atomicAdd(&h.eedf[(size_t)c * h.nCells + i], (unsigned long long)n);
         if (row >= 0 && h.eedfPer)
@@ -665,30 +834,42 @@ __global__ __launch_bounds__(kBlock) void k_hist(DevState<V> s, DevWork w, DevHi
     }
 }
 
-// ----------------------------------------------------------------- reduce
-// summary[c] = [ event sums (recSize) | moment sums (kMomentRecord) | plan (8) | status ]
-__global__ __launch_bounds__(256) void k_reduce(DevWork w, int P) {
+// ----------------------------------------------------------------- finish
+// summary[c] = [ event record (3P + 8) | moment sums (kMomentRecord) | plan (8) ]
+// Chunk records are summed in a fixed order: thread t takes chunks t, t+256,
+// ..., then the block sums the threads in order.  The integer accumulators
+// are converted and cleared for the next interval.  Block 0 resets the
+// flight-kernel work queues.
+__global__ __launch_bounds__(256) void k_finish(DevWork w, int P) {
+    __shared__ double red[256 / kWave];
     const int c = blockIdx.x;
+    if (c == 0 && threadIdx.x < 8) w.queue[threadIdx.x] = 0;
     const CondParams cp = w.conds[c];
-    double* out = w.summary + (size_t)c * w.summarySize;
     if (!cp.active) return;
+    double* out = w.summary + (size_t)c * w.summarySize;
     const int R = w.recSize;
-    for (int f = threadIdx.x; f < R + kMomentRecord; f += blockDim.x) {
+    for (int f = 0; f < CT_SIZE + kMomentRecord; ++f) {
+        const bool isMax = f == CT_MAXE || f == CT_SIZE + kMomentRecord - 1;
         double acc = 0;
-        if (f < R) {
-            bool isMax = f == 3 * P + EV_MAXE;
-            for (int bb = 0; bb < cp.blocks; ++bb) {
-                double v = w.evPart[(size_t)(cp.blockBegin + bb) * R + f];
-                acc = isMax ? fmax(acc, v) : acc + v;
-            }
-        } else {
-            int k = f - R;
-            for (int bb = 0; bb < cp.blocks; ++bb) {
-                double v = w.momPart[(size_t)(cp.blockBegin + bb) * kMomentRecord + k];
-                acc = (k == 26) ? fmax(acc, v) : acc + v;
-            }
+        for (int k = threadIdx.x; k < cp.nChunks; k += blockDim.x) {
+            const size_t j = (size_t)cp.chunkBegin + k;
+            const double v = f < CT_SIZE ? w.tailC[j * CT_SIZE + f] : w.momC[j * kMomentRecord + (f - CT_SIZE)];
+            acc = isMax ? fmax(acc, v) : acc + v;
         }
-        out[f] = acc;
+        const double tot = isMax ? block_max(acc, red) : block_sum(acc, red);
+        if (threadIdx.x == 0) {
+            if (f < CT_SIZE) out[3 * P + f] = tot;
+            else out[R + (f - CT_SIZE)] = tot;
+        }
+    }
+    for (int p = threadIdx.x; p < P; p += blockDim.x) {
+        const size_t i = (size_t)c * P + p;
+        out[p] = (double)w.accCount[i];
+        out[P + p] = (double)w.accGain[i] / kFix;
+        out[2 * P + p] = (double)w.accLoss[i] / kFix;
+        w.accCount[i] = 0;
+        w.accGain[i] = 0;
+        w.accLoss[i] = 0;
     }
     if (threadIdx.x == 0) {
         const PopPlan pl = w.plans[c];
@@ -759,45 +940,67 @@ __global__ __launch_bounds__(kBlock) void k_op_philox(int64_t n, const uint32_t*
 
 // ------------------------------------------------------------- launchers
 static inline unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+static inline unsigned chunk_grid(int nChunks) { return (unsigned)((nChunks + kWavesPerBlock - 1) / kWavesPerBlock); }
+static inline size_t flight_lds(int P) { return (size_t)kWavesPerBlock * 3 * P * 8; }
 
 template <typename V>
 hipError_t launch_init(const DevState<V>& s, const DevWork& w, int64_t totalMain, const int64_t* mainBase,
-                       const double* thermalDev, const int32_t* slotCond, hipStream_t st) {
+                       const double* thermalDev, hipStream_t st) {
     if (totalMain == 0) return hipSuccess;
     timer_start(K_INIT, st);
-    hipLaunchKernelGGL(k_init<V>, dim3(grid_of(totalMain)), dim3(kBlock), 0, st, s, w.conds, totalMain,
-                       mainBase, thermalDev, slotCond);
+    hipLaunchKernelGGL(k_init<V>, dim3(grid_of(totalMain)), dim3(kBlock), 0, st, s, w.conds, w.nCond, totalMain,
+                       mainBase, thermalDev);
     timer_stop(K_INIT, st);
     return hipGetLastError();
 }
 
+template <typename V, int FC, bool TH>
+static const void* flight_kernel() { return (const void*)k_advance<V, FC, TH>; }
+
 template <typename V, int FC>
 static void launch_advance_case(const DevState<V>& s, const DevWork& w, const DevProcesses& pr, const V* tables,
-                                size_t lds, hipStream_t st) {
-    dim3 g(w.nBlocks), b(kBlock);
+                                const FlightLists& fl, hipStream_t st) {
+    if (fl.count[FC] == 0) return;
+    dim3 g(fl.grid[FC]), b(kBlock);
+    const size_t lds = flight_lds(pr.P);
+    uint32_t* q = w.queue + FC;
     if (pr.gasTempEffect == 0)
-        hipLaunchKernelGGL((k_advance<V, FC, false>), g, b, lds, st, s, w, pr, tables);
+        hipLaunchKernelGGL((k_advance<V, FC, false>), g, b, lds, st, s, w, pr, tables, fl.list[FC], fl.count[FC], q);
     else
-        hipLaunchKernelGGL((k_advance<V, FC, true>), g, b, lds, st, s, w, pr, tables);
+        hipLaunchKernelGGL((k_advance<V, FC, true>), g, b, lds, st, s, w, pr, tables, fl.list[FC], fl.count[FC], q);
+}
+
+template <typename V>
+int flight_blocks_per_cu(int fc, bool thermal, int P) {
+    const void* k = nullptr;
+    switch (fc) {
+        case FIELD_DC: k = thermal ? flight_kernel<V, FIELD_DC, true>() : flight_kernel<V, FIELD_DC, false>(); break;
+        case FIELD_AC: k = thermal ? flight_kernel<V, FIELD_AC, true>() : flight_kernel<V, FIELD_AC, false>(); break;
+        case FIELD_DC_B: k = thermal ? flight_kernel<V, FIELD_DC_B, true>() : flight_kernel<V, FIELD_DC_B, false>(); break;
+        case FIELD_ECR: k = thermal ? flight_kernel<V, FIELD_ECR, true>() : flight_kernel<V, FIELD_ECR, false>(); break;
+        default: k = thermal ? flight_kernel<V, FIELD_AC_B, true>() : flight_kernel<V, FIELD_AC_B, false>(); break;
+    }
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kBlock, flight_lds(P)) != hipSuccess || n < 1) n = 1;
+    return n;
 }
 
 template <typename V>
 hipError_t launch_interval(const DevState<V>& s, const DevWork& w, const DevProcesses& pr, const V* tables,
-                           uint32_t caseMask, hipStream_t st) {
-    size_t lds = (size_t)3 * pr.P * 8 + (kBlock / kWave) * 8;
-    dim3 g(w.nBlocks), b(kBlock);
+                           const FlightLists& fl, hipStream_t st) {
+    const dim3 g(chunk_grid(w.nChunks)), b(kBlock);
     timer_start(K_ADVANCE, st);
-    if (caseMask & (1u << FIELD_DC)) launch_advance_case<V, FIELD_DC>(s, w, pr, tables, lds, st);
-    if (caseMask & (1u << FIELD_AC)) launch_advance_case<V, FIELD_AC>(s, w, pr, tables, lds, st);
-    if (caseMask & (1u << FIELD_DC_B)) launch_advance_case<V, FIELD_DC_B>(s, w, pr, tables, lds, st);
-    if (caseMask & (1u << FIELD_ECR)) launch_advance_case<V, FIELD_ECR>(s, w, pr, tables, lds, st);
-    if (caseMask & (1u << FIELD_AC_B)) launch_advance_case<V, FIELD_AC_B>(s, w, pr, tables, lds, st);
+    launch_advance_case<V, FIELD_DC>(s, w, pr, tables, fl, st);
+    launch_advance_case<V, FIELD_AC>(s, w, pr, tables, fl, st);
+    launch_advance_case<V, FIELD_DC_B>(s, w, pr, tables, fl, st);
+    launch_advance_case<V, FIELD_ECR>(s, w, pr, tables, fl, st);
+    launch_advance_case<V, FIELD_AC_B>(s, w, pr, tables, fl, st);
     timer_stop(K_ADVANCE, st);
     timer_start(K_PLAN, st);
     hipLaunchKernelGGL(k_plan, dim3(w.nCond), dim3(1024), 0, st, w);
     timer_stop(K_PLAN, st);
     timer_start(K_MARKS, st);
-    hipLaunchKernelGGL(k_count_marks<V>, g, b, 0, st, s, w);
+    hipLaunchKernelGGL(k_marks, g, b, 0, st, s.alive, w);
     hipLaunchKernelGGL(k_plan_marks, dim3(w.nCond), dim3(1024), 0, st, w);
     timer_stop(K_MARKS, st);
     timer_start(K_APPLY, st);
@@ -818,23 +1021,24 @@ hipError_t launch_interval(const DevState<V>& s, const DevWork& w, const DevProc
 template <typename V>
 hipError_t launch_sample(const DevState<V>& s, const DevWork& w, hipStream_t st) {
     timer_start(K_SAMPLE, st);
-    hipLaunchKernelGGL(k_sample<V>, dim3(w.nBlocks), dim3(kBlock), 0, st, s, w);
+    hipLaunchKernelGGL(k_sample<V>, dim3(chunk_grid(w.nChunks)), dim3(kBlock), 0, st, s, w);
     timer_stop(K_SAMPLE, st);
     return hipGetLastError();
 }
 
 template <typename V>
 hipError_t launch_hist(const DevState<V>& s, const DevWork& w, const DevHist& h, hipStream_t st) {
-    size_t lds = (size_t)h.nCells * 4;
+    const int ldsCells = h.nCells <= 2048 ? h.nCells : 0;
+    const size_t lds = (size_t)kWavesPerBlock * ldsCells * 4;
     timer_start(K_HIST, st);
-    hipLaunchKernelGGL(k_hist<V>, dim3(w.nBlocks), dim3(kBlock), lds, st, s, w, h);
+    hipLaunchKernelGGL(k_hist<V>, dim3(chunk_grid(w.nChunks)), dim3(kBlock), lds, st, s, w, h, ldsCells);
     timer_stop(K_HIST, st);
     return hipGetLastError();
 }
 
-hipError_t launch_reduce(const DevWork& w, int P, hipStream_t st) {
+hipError_t launch_finish(const DevWork& w, int P, hipStream_t st) {
     timer_start(K_REDUCE, st);
-    hipLaunchKernelGGL(k_reduce, dim3(w.nCond), dim3(256), 0, st, w, P);
+    hipLaunchKernelGGL(k_finish, dim3(w.nCond), dim3(256), 0, st, w, P);
     timer_stop(K_REDUCE, st);
     return hipGetLastError();
 }
@@ -866,11 +1070,12 @@ hipError_t launch_op_philox(int64_t n, const uint32_t* ctr, const uint32_t* key,
 
 #define EKMC_INSTANTIATE(V)                                                                                   \
     template hipError_t launch_init<V>(const DevState<V>&, const DevWork&, int64_t, const int64_t*,           \
-                                       const double*, const int32_t*, hipStream_t);                           \
+                                       const double*, hipStream_t);                                           \
     template hipError_t launch_interval<V>(const DevState<V>&, const DevWork&, const DevProcesses&, const V*, \
-                                           uint32_t, hipStream_t);                                            \
+                                           const FlightLists&, hipStream_t);                                  \
     template hipError_t launch_sample<V>(const DevState<V>&, const DevWork&, hipStream_t);                    \
     template hipError_t launch_hist<V>(const DevState<V>&, const DevWork&, const DevHist&, hipStream_t);      \
+    template int flight_blocks_per_cu<V>(int, bool, int);                                                     \
     template hipError_t launch_op_accelerate<V>(const CondParams*, int64_t, const double*, const double*,     \
                                                 double*, double*, double*, double*, hipStream_t);             \
     template hipError_t launch_op_collide<V>(const CondParams*, const DevProcesses&, const V*, int64_t,       \

@@ -79,7 +79,8 @@ class Numerics(C.Structure):
                 ("rel_error_power_balance", C.c_double), ("n_integration_points", C.c_double),
                 ("n_integrated_ss_times", C.c_double), ("integrated_absolute_time", C.c_double),
                 ("sync_time_x_max_coll_freq", C.c_double), ("sync_over_sampling", C.c_int32),
-                ("max_sync_points", C.c_int32), ("seed", C.c_uint64)]
+                ("max_sync_points", C.c_int32), ("seed", C.c_uint64),
+                ("work_chunk", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Results(C.Structure):
