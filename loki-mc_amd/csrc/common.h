@@ -13,7 +13,6 @@ constexpr double kPi = 3.14159265358979323846;
 
 constexpr int kBlock = 256;            // threads per block of every grid-wide kernel
 constexpr int kWave = 64;              // CDNA wavefront
-constexpr int kMaxSkip = 4096;         // capacity of the removal-correction set per condition
 constexpr int kChunkDefault = 1024;    // main electrons per wave work unit (64 lanes x 16)
 constexpr uint32_t kMaxLoopPerSync = 1u << 22;  // flight cap per electron per interval
 
@@ -28,7 +27,7 @@ enum StatusBits : uint32_t {
     ST_REGION_OVERFLOW = 1u,   // a chunk produced more newborns than its region holds
     ST_ZERO_VELOCITY = 2u,     // cart2sph of a null vector
     ST_LOOP_CAP = 4u,          // flight cap reached inside one interval
-    ST_POPCTL = 8u,            // removal correction set too large
+    ST_POPCTL = 8u,            // population control could not complete (marks < removals)
     ST_NONFINITE = 16u,        // an electron clock or energy became NaN/inf
 };
 
@@ -85,10 +84,10 @@ struct PopPlan {
     int64_t alive;       // M
     int64_t remove;      // R = max(0, M - N)
     int64_t marks;       // K
-    int64_t skips;       // K - R
+    int64_t thrIndex;    // removal threshold: mark j goes iff (key_j, j) <= (thrKey, thrIndex)
     double markProb;     // p
     uint32_t status;
-    uint32_t pad;
+    uint32_t thrKey;
 };
 
 // Per-condition event record of one interval (summary of the flight and

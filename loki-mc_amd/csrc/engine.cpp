@@ -194,7 +194,7 @@ public:
     std::vector<Chunk> hchunks;
     // device buffers
     DevBuf<Rec<V>> rec;
-    DevBuf<uint32_t> holeList, skipSet, status, queue;
+    DevBuf<uint32_t> holeList, status, queue;
     DevBuf<uint8_t> alive;
     DevBuf<V> tables;
     DevBuf<CondParams> dconds;
@@ -227,7 +227,7 @@ public:
             if (timer.start[k]) (void)hipEventDestroy(timer.start[k]);
             if (timer.stop[k]) (void)hipEventDestroy(timer.stop[k]);
         }
-        rec.release(); holeList.release(); skipSet.release(); status.release(); queue.release();
+        rec.release(); holeList.release(); status.release(); queue.release();
         alive.release(); tables.release(); dconds.release(); chunks.release(); holesC.release();
         surplusC.release(); markC.release(); flist.release(); phaseRow.release(); cylFlag.release();
         holeOff.release(); surOff.release(); markOff.release(); mainBase.release(); tailC.release();
@@ -383,7 +383,6 @@ public:
         HIP_TRY(queue.alloc(8));
         HIP_TRY(hipMemsetAsync(queue.p, 0, 8 * sizeof(uint32_t), stream));
         HIP_TRY(plans.alloc(n));
-        HIP_TRY(skipSet.alloc((size_t)n * kMaxSkip));
         HIP_TRY(status.alloc(n));
         HIP_TRY(hipMemsetAsync(status.p, 0, n * sizeof(uint32_t), stream));
         HIP_TRY(hipMemsetAsync(plans.p, 0, n * sizeof(PopPlan), stream));
@@ -428,7 +427,7 @@ public:
         dw.conds = dconds.p; dw.chunks = chunks.p; dw.holesC = holesC.p; dw.surplusC = surplusC.p;
         dw.markC = markC.p; dw.holeOff = holeOff.p; dw.surOff = surOff.p; dw.markOff = markOff.p;
         dw.holeList = holeList.p; dw.tailC = tailC.p; dw.momC = momC.p; dw.accCount = accCount.p;
-        dw.accGain = accGain.p; dw.accLoss = accLoss.p; dw.plans = plans.p; dw.skipSet = skipSet.p;
+        dw.accGain = accGain.p; dw.accLoss = accLoss.p; dw.plans = plans.p;
         dw.status = status.p; dw.queue = queue.p; dw.summary = summary.p; dw.summarySize = summarySize;
         dw.recSize = recSize; dw.nChunks = nChunks; dw.nCond = n;
         dh.eedf = hEedf.p; dh.eedfPer = hPer.p; dh.eadf = hEadf.p; dh.evdf = hEvdf.p; dh.grid = grid.p;

@@ -66,7 +66,6 @@ struct DevWork {
     long long* accGain;           // [n_cond * P] fixed-point energy gains (2^-24 eV)
     long long* accLoss;           // [n_cond * P]
     PopPlan* plans;               // [n_cond]
-    uint32_t* skipSet;            // [n_cond * kMaxSkip]
     uint32_t* status;             // [n_cond]
     uint32_t* queue;              // [8] work-queue counters of the flight kernels
     double* summary;              // [n_cond * summarySize]

@@ -15,7 +15,8 @@
 //                  population M, removals R or clones C.
 //   k_marks / k_plan_marks / k_apply_marks
 //                  exact uniform removal of R electrons: Bernoulli marks drawn
-//                  by geometric skips, then a Floyd sample of marks to keep.
+//                  by geometric skips, then the R marks with the smallest
+//                  random keys (radix select) are removed.
 //   k_plan_pairs / k_move / k_clone
 //                  surplus newborns fill holes; clones of uniformly chosen
 //                  electrons fill the holes that remain (rejection, as the
@@ -523,56 +524,88 @@ __global__ __launch_bounds__(kBlock) void k_marks(const uint8_t* alive, DevWork 
     if (lane == 0) w.markC[cid] = (int32_t)m;
 }
 
-// Scan of marks and Floyd's sample of the K - R marks that are kept alive.
+// Key of the mark with condition-wide index j (uniform 32-bit word).
+__device__ __forceinline__ uint32_t mark_key(const CondParams& cp, int64_t j) {
+    uint32_t w[4];
+    philox_call(cp.seed, (uint64_t)j, (uint32_t)cp.syncIndex, P_POPCTL,
+                0x8000u | ((uint32_t)(cp.syncIndex >> 32) & 0x7fffu), w);
+    return w[0];
+}
+
+// Scan of marks, then the removal threshold: of the K marks, the R with the
+// smallest (key, index) go, which is a uniform R-subset.  The R-th smallest
+// key comes from a 4-pass, 8-bit radix select; ties on that key are ordered
+// by index.
+constexpr int kTieCap = 1024;
 __global__ __launch_bounds__(1024) void k_plan_marks(DevWork w) {
     __shared__ int64_t lds[1024];
-    __shared__ uint32_t hashset[2 * kMaxSkip];
-    __shared__ uint32_t chosen[kMaxSkip];
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t ties[kTieCap];
+    __shared__ uint32_t sh[4];  // prefix, mask, need, tie count
     const int c = blockIdx.x;
     const CondParams cp = w.conds[c];
     if (!cp.active) return;
     PopPlan pl = w.plans[c];
     if (pl.remove == 0) return;
-    int64_t K = block_exclusive_scan(w.markC + cp.chunkBegin, w.markOff + cp.chunkBegin, cp.nChunks, lds);
-    int64_t m = K - pl.remove;
-    bool bad = K < pl.remove || m > kMaxSkip;
-    if (threadIdx.x == 0) {
-        pl.marks = K;
-        pl.skips = bad ? 0 : m;
-        if (bad) pl.status |= ST_POPCTL;
-        w.plans[c] = pl;
-        if (bad) atomicOr(&w.status[c], (uint32_t)ST_POPCTL);
+    const int64_t K = block_exclusive_scan(w.markC + cp.chunkBegin, w.markOff + cp.chunkBegin, cp.nChunks, lds);
+    const int64_t R = pl.remove;
+    if (K < R || K > 0xffffffffLL) {
+        if (threadIdx.x == 0) {
+            pl.marks = K;
+            pl.status |= ST_POPCTL;
+            w.plans[c] = pl;
+            atomicOr(&w.status[c], (uint32_t)ST_POPCTL);
+        }
+        return;
     }
-    if (bad || m == 0) return;
-    for (int i = threadIdx.x; i < 2 * kMaxSkip; i += blockDim.x) hashset[i] = 0xffffffffu;
+    if (threadIdx.x == 0) { sh[0] = 0; sh[1] = 0; sh[2] = (uint32_t)R; sh[3] = 0; }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        // Floyd: for j = K-m .. K-1, t ~ U{0..j}; insert t, or j if t is taken
-        int n = 0;
-        const uint32_t mask = 2 * kMaxSkip - 1;
-        for (int64_t j = K - m; j < K; ++j) {
-            double u = u53(cp.seed, (uint64_t)j, (uint32_t)cp.syncIndex, P_POPCTL,
-                           0x8000u | ((uint32_t)(cp.syncIndex >> 32) & 0x7fffu));
-            uint32_t t = (uint32_t)fmin(floor(u * (double)(j + 1)), (double)j);
-            uint32_t pick = t;
-            for (int pass = 0; pass < 2; ++pass) {
-                uint32_t h = (pick * 2654435761u) & mask;
-                bool found = false;
-                while (hashset[h] != 0xffffffffu) {
-                    if (hashset[h] == pick) { found = true; break; }
-                    h = (h + 1) & mask;
-                }
-                if (!found) { hashset[h] = pick; chosen[n++] = pick; break; }
-                pick = (uint32_t)j;  // t taken: take j (never taken before)
+    for (int pass = 0; pass < 4; ++pass) {
+        const int shift = 24 - 8 * pass;
+        for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+        __syncthreads();
+        const uint32_t prefix = sh[0], mask = sh[1];
+        for (int64_t j = threadIdx.x; j < K; j += blockDim.x) {
+            const uint32_t k = mark_key(cp, j);
+            if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t need = sh[2], cum = 0;
+            int bkt = 0;
+            for (; bkt < 255; ++bkt) {
+                if (cum + hist[bkt] >= need) break;
+                cum += hist[bkt];
             }
+            sh[0] = prefix | ((uint32_t)bkt << shift);
+            sh[1] = mask | (255u << shift);
+            sh[2] = need - cum;
+        }
+        __syncthreads();
+    }
+    const uint32_t T = sh[0], need = sh[2];  // R-th smallest key; its rank among keys == T
+    for (int64_t j = threadIdx.x; j < K; j += blockDim.x) {
+        if (mark_key(cp, j) == T) {
+            const uint32_t slot = atomicAdd(&sh[3], 1u);
+            if (slot < kTieCap) ties[slot] = (uint32_t)j;
         }
     }
     __syncthreads();
-    // bitonic sort of the m chosen indices (padded to a power of two)
+    const uint32_t nt = sh[3];
+    if (nt > kTieCap || need < 1 || need > nt) {
+        if (threadIdx.x == 0) {
+            pl.marks = K;
+            pl.status |= ST_POPCTL;
+            w.plans[c] = pl;
+            atomicOr(&w.status[c], (uint32_t)ST_POPCTL);
+        }
+        return;
+    }
+    // bitonic sort of the tied indices (padded to a power of two)
     int n2 = 1;
-    while (n2 < m) n2 <<= 1;
+    while (n2 < (int)nt) n2 <<= 1;
     for (int i = threadIdx.x; i < n2; i += blockDim.x)
-        if (i >= m) chosen[i] = 0xffffffffu;
+        if (i >= (int)nt) ties[i] = 0xffffffffu;
     __syncthreads();
     for (int k = 2; k <= n2; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
@@ -580,27 +613,23 @@ __global__ __launch_bounds__(1024) void k_plan_marks(DevWork w) {
                 int l = i ^ j;
                 if (l > i) {
                     bool up = (i & k) == 0;
-                    uint32_t a = chosen[i], bb = chosen[l];
-                    if ((a > bb) == up) { chosen[i] = bb; chosen[l] = a; }
+                    uint32_t a = ties[i], bb = ties[l];
+                    if ((a > bb) == up) { ties[i] = bb; ties[l] = a; }
                 }
             }
             __syncthreads();
         }
     }
-    for (int i = threadIdx.x; i < m; i += blockDim.x) w.skipSet[(size_t)c * kMaxSkip + i] = chosen[i];
-}
-
-__device__ __forceinline__ bool in_sorted(const uint32_t* a, int64_t n, uint32_t v) {
-    int64_t lo = 0, hi = n;
-    while (lo < hi) {
-        int64_t mid = (lo + hi) >> 1;
-        if (a[mid] < v) lo = mid + 1; else hi = mid;
+    if (threadIdx.x == 0) {
+        pl.marks = K;
+        pl.thrKey = T;
+        pl.thrIndex = ties[need - 1];
+        w.plans[c] = pl;
     }
-    return lo < n && a[lo] == v;
 }
 
-// Marks in chunk order, then lane order, then position order; a mark whose
-// index is not in the kept set is removed.  Removed main electrons become
+// Marks in chunk order, then lane order, then position order; a mark at or
+// below the threshold is removed.  Removed main electrons become
 // holes of the chunk, then the chunk refills them from its newborns.
 template <typename V>
 __global__ __launch_bounds__(kBlock) void k_apply_marks(DevState<V> s, DevWork w) {
@@ -612,7 +641,6 @@ __global__ __launch_bounds__(kBlock) void k_apply_marks(DevState<V> s, DevWork w
     if (!cp.active) return;
     const PopPlan pl = w.plans[ch.cond];
     if (pl.remove == 0 || (pl.status & ST_POPCTL)) return;
-    const uint32_t* skip = w.skipSet + (size_t)ch.cond * kMaxSkip;
     const int sur = w.surplusC[cid];
     const int64_t L = (int64_t)ch.count + sur;
     int mine = 0;
@@ -630,7 +658,8 @@ __global__ __launch_bounds__(kBlock) void k_apply_marks(DevState<V> s, DevWork w
         int64_t slot = 0;
         if (more) {
             if (mark_alive(s.alive, ch, g.pos)) {
-                if (!in_sorted(skip, pl.skips, (uint32_t)gi)) {
+                const uint32_t k = mark_key(cp, gi);
+                if (k < pl.thrKey || (k == pl.thrKey && gi <= pl.thrIndex)) {
                     kill = true;
                     killMain = g.pos < ch.count;
                     slot = killMain ? ch.first + g.pos : ch.region + (g.pos - ch.count);
@@ -875,7 +904,7 @@ __global__ __launch_bounds__(256) void k_finish(DevWork w, int P) {
         const PopPlan pl = w.plans[c];
         double* tail = out + R + kMomentRecord;
         tail[0] = (double)pl.holes; tail[1] = (double)pl.surplus; tail[2] = (double)pl.alive;
-        tail[3] = (double)pl.remove; tail[4] = (double)pl.marks; tail[5] = (double)pl.skips;
+        tail[3] = (double)pl.remove; tail[4] = (double)pl.marks; tail[5] = (double)pl.thrIndex;
         tail[6] = (double)w.status[c];
         tail[7] = 0;
     }

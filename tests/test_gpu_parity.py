@@ -253,6 +253,33 @@ def test_population_control_keeps_n_and_balances_power(wl, idx):
     eng.close()
 
 
+@pytest.mark.parametrize("prec,n", [(ekmc.FP64, 300000), (ekmc.MIXED, 12000000)])
+def test_large_removal_sets_keep_n(prec, n):
+    """1000 Td: about 2.5 % of the electrons ionize per interval, far more than
+    attach, so that many electrons are removed each interval through the
+    radix-selected threshold.  At 1.2e7 electrons that is ~3e5 removals per
+    interval (a Floyd correction set of 8 sqrt(R) would hold > 4096 entries).
+    N stays exact and the run is bitwise reproducible."""
+    mix, conds, _ = ekmc.workload("cfg3_o2like_batched", n_electrons=n, conditions=[63])
+    ion = np.flatnonzero(np.asarray(mix.arrays["type"]) == 1)
+    out = []
+    for _ in range(2):
+        num = _short_numerics(3, prec)
+        num.max_collisions_before_ss = 1e9
+        eng = _engine(mix, conds, num)
+        eng.initialize()
+        eng.step(40)
+        rates, counts, gain, loss = eng.process_data(0)
+        pos, vel, e, t = eng.electrons(0)
+        out.append(vel[::97].copy())
+        eng.close()
+    per_interval = counts[ion].sum() / 40.0
+    print("ionizations per interval", per_interval)
+    assert per_interval > 0.02 * n
+    assert len(e) == n and np.all(np.isfinite(e)) and np.all(t == t[0])
+    assert np.array_equal(out[0], out[1])
+
+
 def _combined_sigma(a, ea, b, eb, k=4.0):
     return abs(a - b) <= k * np.hypot(ea, eb)
 
