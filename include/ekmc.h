@@ -302,7 +302,12 @@ const char* ekmc_last_error(void);
  *  5 channel choice U(0,1]     6 scattered cos(chi) sampler U[0,1]
  *  7 scattered azimuth U(0,1]  8 ionization energy share U[0,1]
  *  9 ejected cos(chi) U[0,1]   10 ejected azimuth U(0,1]
- * (draw order of BoltzmannMC.C:652-674,907-1272 and MathFunctions.C:54-59) */
+ * (draw order of BoltzmannMC.C:652-674,907-1272 and MathFunctions.C:54-59).
+ * In the engine, slot s of event k of an electron with stream id S comes from
+ * Philox4x32-10 with counter {S lo, S hi, k, call} and key = seed:
+ *   FP64: 53 bits of words (0,1) or (2,3); call 0 {0,5}, 1 {6,7}, 2 {1,2},
+ *         3 {3,4}, 4 {8,9}, 5 {10}
+ *   FP32: top 24 bits of one word; call 0 {0,5,6,7}, 1 {1,2,3,4}, 2 {8,9,10} */
 #define EKMC_VARIATES_PER_EVENT 11
 
 /* Collision tables of one condition as the engine built them

@@ -62,6 +62,7 @@ struct CondParams {
     double gasEnergy;        // 1.5 k Tg / e [eV]
     // per interval
     double trial;            // trial collision frequency [1/s]
+    double trialInv;         // 1 / trial
     double tsync;            // synchronization time of this interval [s]
     double step;             // table energy step [eV]
     uint64_t seed;           // Philox key of this condition

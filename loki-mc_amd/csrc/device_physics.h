@@ -15,13 +15,15 @@ namespace ekmc {
 
 // ------------------------------------------------------------------ Philox
 // Philox4x32-10 (Salmon et al., SC'11): ten rounds, Weyl key bumps between.
+// Each round's two 32x32-bit products are formed as full 64-bit products.
 __device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-        uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
-        uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
-        uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+        const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
         c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
     }
 }
@@ -41,52 +43,98 @@ __device__ __forceinline__ uint64_t child_stream(uint64_t seed, uint64_t parent,
 }
 
 // Slot intervals (ekmc.h): 0 (0,1); 1-5,7,10 (0,1]; 6,8,9 [0,1].
-__device__ __forceinline__ int slot_kind(int s) {
+__device__ __forceinline__ constexpr int slot_kind(int s) {
     return s == 0 ? 0 : ((s == 6 || s == 8 || s == 9) ? 2 : 1);
 }
 
-template <typename V> struct EventRng;
+// Slot -> Philox call of the event.  Call 0 carries everything a cold
+// collision event needs (free time, channel choice, cos(chi), azimuth), so a
+// flight that ends in a collision costs one call; thermal and ionization
+// slots come from later calls.
+//   FP64 (one slot = 53 bits of two words, two slots per call):
+//     call 0 {0,5}  1 {6,7}  2 {1,2}  3 {3,4}  4 {8,9}  5 {10}
+//   FP32 (one slot = the top 24 bits of one word, four slots per call):
+//     call 0 {0,5,6,7}  1 {1,2,3,4}  2 {8,9,10}
+__device__ __forceinline__ constexpr int call64(int s) {
+    return (s == 0 || s == 5) ? 0 : (s == 6 || s == 7) ? 1 : (s <= 2) ? 2 : (s <= 4) ? 3 : (s <= 9) ? 4 : 5;
+}
+__device__ __forceinline__ constexpr int half64(int s) {
+    return (s == 5 || s == 7 || s == 2 || s == 4 || s == 9) ? 1 : 0;
+}
+__device__ __forceinline__ constexpr int call32(int s) {
+    return (s == 0 || (s >= 5 && s <= 7)) ? 0 : (s <= 4 ? 1 : 2);
+}
+__device__ __forceinline__ constexpr int word32(int s) {
+    return s == 0 ? 0 : (s <= 4 ? s - 1 : (s <= 7 ? s - 4 : s - 8));
+}
 
-// FP64: slot s is the 53-bit integer of words (2h,2h+1) of call s/2.
-template <> struct EventRng<double> {
+// Variates of one event: call 0 cached in w0, one other call in w1.  With a
+// tape (batch operators) the slot values are read from it instead.
+template <typename V> struct EventRng {
     uint64_t seed, stream;
     uint32_t event;
-    int cached;
-    uint32_t w[4];
+    bool have0;
+    int other;
+    uint32_t w0[4], w1[4];
     const double* tape;
     __device__ EventRng(uint64_t sd, uint64_t st, uint32_t ev, const double* tp = nullptr)
-        : seed(sd), stream(st), event(ev), cached(-1), tape(tp) {}
-    __device__ double u(int s) {
-        if (tape) return tape[s];
-        int call = s >> 1;
-        if (call != cached) { philox_call(seed, stream, event, P_EVENT, (uint32_t)call, w); cached = call; }
-        int h = s & 1;
-        uint64_t x = ((((uint64_t)w[2 * h]) << 32) | w[2 * h + 1]) >> 11;
-        const double k = 1.0 / 9007199254740992.0;
-        int kind = slot_kind(s);
-        return kind == 0 ? ((double)x + 0.5) * k : (kind == 1 ? ((double)x + 1.0) * k : (double)x * k);
+        : seed(sd), stream(st), event(ev), have0(false), other(-1), tape(tp) {}
+    // with call 0 already drawn
+    __device__ EventRng(uint64_t sd, uint64_t st, uint32_t ev, const uint32_t c0[4])
+        : seed(sd), stream(st), event(ev), have0(true), other(-1), tape(nullptr) {
+        w0[0] = c0[0]; w0[1] = c0[1]; w0[2] = c0[2]; w0[3] = c0[3];
     }
+    __device__ __forceinline__ void prime0() {
+        if (!have0) { philox_call(seed, stream, event, P_EVENT, 0u, w0); have0 = true; }
+    }
+    __device__ __forceinline__ const uint32_t* words(int call) {
+        if (call == 0) { prime0(); return w0; }
+        if (other != call) { philox_call(seed, stream, event, P_EVENT, (uint32_t)call, w1); other = call; }
+        return w1;
+    }
+    __device__ __forceinline__ V u(int s);
 };
 
-// FP32: slot s is the top 24 bits of word s%4 of call s/4.
-template <> struct EventRng<float> {
-    uint64_t seed, stream;
-    uint32_t event;
-    int cached;
-    uint32_t w[4];
-    const double* tape;
-    __device__ EventRng(uint64_t sd, uint64_t st, uint32_t ev, const double* tp = nullptr)
-        : seed(sd), stream(st), event(ev), cached(-1), tape(tp) {}
-    __device__ float u(int s) {
-        if (tape) return (float)tape[s];
-        int call = s >> 2;
-        if (call != cached) { philox_call(seed, stream, event, P_EVENT, (uint32_t)call, w); cached = call; }
-        float x = (float)(w[s & 3] >> 8);
-        const float k = 1.0f / 16777216.0f;
-        int kind = slot_kind(s);
-        return kind == 0 ? (x + 0.5f) * k : (kind == 1 ? (x + 1.0f) * k : x * k);
-    }
-};
+template <> __device__ __forceinline__ double EventRng<double>::u(int s) {
+    if (tape) return tape[s];
+    const uint32_t* w = words(call64(s));
+    const int h = half64(s);
+    const uint32_t a = h ? w[2] : w[0], b = h ? w[3] : w[1];
+    const uint64_t x = ((((uint64_t)a) << 32) | b) >> 11;
+    const double k = 1.0 / 9007199254740992.0;
+    const int kind = slot_kind(s);
+    return kind == 0 ? ((double)x + 0.5) * k : (kind == 1 ? ((double)x + 1.0) * k : (double)x * k);
+}
+
+template <> __device__ __forceinline__ float EventRng<float>::u(int s) {
+    if (tape) return (float)tape[s];
+    const uint32_t* w = words(call32(s));
+    const int q = word32(s);
+    const uint32_t v = q == 0 ? w[0] : q == 1 ? w[1] : q == 2 ? w[2] : w[3];
+    const float x = (float)(v >> 8);
+    const float k = 1.0f / 16777216.0f;
+    const int kind = slot_kind(s);
+    return kind == 0 ? (x + 0.5f) * k : (kind == 1 ? (x + 1.0f) * k : x * k);
+}
+
+// Free time -ln(U)/nu_trial (BoltzmannMC.C:652-653): FP64 in the reference
+// order; FP32 with the hardware logarithm and the reciprocal frequency.
+__device__ __forceinline__ double free_time(double u, const CondParams& c) { return -log(u) / c.trial; }
+__device__ __forceinline__ float free_time(float u, const CondParams& c) {
+    return -__logf(u) * (float)c.trialInv;
+}
+
+// sin and cos of 2 pi u (azimuths): FP64 as the reference computes them;
+// FP32 with the hardware sine/cosine, whose argument is in turns.
+__device__ __forceinline__ void turn_sincos(double u, double& s, double& co) {
+    const double a = (2.0 * kPi) * u;
+    s = sin(a); co = cos(a);
+}
+__device__ __forceinline__ void turn_sincos(float u, float& s, float& co) {
+    s = __builtin_amdgcn_sinf(u); co = __builtin_amdgcn_cosf(u);
+}
+__device__ __forceinline__ double log_of(double x) { return log(x); }
+__device__ __forceinline__ float log_of(float x) { return __logf(x); }
 
 // ---------------------------------------------------------------- electron
 template <typename V> struct Electron {
@@ -280,7 +328,8 @@ __device__ inline void flight(const CondParams& c, Electron<float>& e, double dt
 // --------------------------------------------------------------- samplers
 // AngularScatteringFunctions.h:27-74; `slot` is the variate slot.
 template <typename V>
-__device__ inline V cos_chi(const DevProcesses& pr, int p, V energy, V energyAfter, EventRng<V>& rng, int slot) {
+__device__ __forceinline__ V cos_chi(const DevProcesses& pr, int p, V energy, V energyAfter, EventRng<V>& rng,
+                                    int slot) {
     switch (pr.scat[p]) {
         case 1: return (V)1;  // forward
         case 2: {             // bornDipole
@@ -364,69 +413,108 @@ __device__ inline int channel_search(const V* c1, const V* c2, const V* i1, cons
 
 // performCollision (BoltzmannMC.C:907-1113) with conservativeCollision
 // (:1115-1193), ionizationCollision (:1195-1272) and attachmentCollision
-// (:1274-1280).  `tab` points at this condition's tables: cumul [I*P],
-// interp [I*P], nutot [I].
-template <typename V, bool THERMAL>
-__device__ inline void collide(const CondParams& c, const DevProcesses& pr, const V* cumul, const V* interp,
-                               const V* nutot, Electron<V>& e, EventRng<V>& rng, CollOut<V>& o) {
-    const int P = pr.P, I = c.I;
-    o.pid = -1; o.flags = 0; o.ejEps = (V)0; o.dE = 0; o.escape = false;
-    o.ejv[0] = o.ejv[1] = o.ejv[2] = (V)0;
-    V Vt[3] = {(V)0, (V)0, (V)0};
-    int chosen = -1;
+// (:1274-1280), in parts: the cold null test, the cold and thermal channel
+// choices, and the collision operator of the chosen process.  `cumul`,
+// `interp` [I*P] and `nutot` [I] are this condition's tables.
+
+// Thermal target motion for this electron (BoltzmannMC.C:916)
+template <typename V>
+__device__ __forceinline__ bool is_thermal(const CondParams& c, const DevProcesses& pr, const Electron<V>& e) {
     const int gte = pr.gasTempEffect;
-    const bool thermal = THERMAL && (gte == 1 || (gte == 2 && (double)e.eps < 20.0 * c.gasEnergy));
+    return gte == 1 || (gte == 2 && (double)e.eps < 20.0 * c.gasEnergy);
+}
+
+// Table rows and weights of the cold branch (BoltzmannMC.C:1035-1047)
+template <typename V> struct ColdRow {
+    int k1, k2;
+    V w1, w2;
+};
+template <typename V> __device__ __forceinline__ ColdRow<V> cold_row(const CondParams& c, const Electron<V>& e) {
+    ColdRow<V> r;
+    const V over = e.eps / (V)c.step;
+    r.k1 = (int)fmin(over, (V)(c.I - 1));
+    r.k2 = (int)fmin((V)(r.k1 + 1), (V)(c.I - 1));
+    r.w1 = (V)r.k2 - over;
+    if (r.w1 < (V)0) r.w1 = (V)0;
+    r.w2 = (V)1 - r.w1;
+    return r;
+}
+
+// Null test of the cold branch (BoltzmannMC.C:1049-1053): true for a real
+// collision.  Also flags a flight whose trial frequency was below the total.
+template <typename V>
+__device__ __forceinline__ bool cold_null_test(const CondParams& c, const V* nutot, const Electron<V>& e,
+                                               EventRng<V>& rng, CollOut<V>& o) {
+    const ColdRow<V> r = cold_row(c, e);
+    const V randF = e.nue * rng.u(5);
+    const V tot = lerp2(r.w1, nutot[r.k1], r.w2, nutot[r.k2]);
+    o.escape = e.nue < tot;
+    return !(randF > tot);
+}
+
+// Cold channel choice of a real collision (BoltzmannMC.C:1055-1097)
+template <typename V>
+__device__ __forceinline__ int cold_choose(const CondParams& c, const DevProcesses& pr, const V* cumul,
+                                           const V* interp, const Electron<V>& e, EventRng<V>& rng) {
+    const int P = pr.P;
+    const ColdRow<V> r = cold_row(c, e);
+    const V randF = e.nue * rng.u(5);
+    const V* c1 = cumul + (size_t)r.k1 * P;
+    const V* c2 = cumul + (size_t)r.k2 * P;
+    const V* i1 = interp + (size_t)r.k1 * P;
+    const V* i2 = interp + (size_t)r.k2 * P;
+    const V speed = sqrt(sqr_norm(e.vx, e.vy, e.vz));
+    const V randCS = randF / (V)c.N / speed;
+    return channel_search(c1, c2, i1, i2, r.w1, r.w2, 0, P - 1, randCS, (V)0, (V)0, speed, false);
+}
+
+// Thermal channel choice (BoltzmannMC.C:916-1031): target velocity Vt from
+// three normals; -1 for a null collision.
+template <typename V>
+__device__ __forceinline__ int thermal_choose(const CondParams& c, const DevProcesses& pr, const V* cumul,
+                                              const V* interp, const Electron<V>& e, EventRng<V>& rng, V Vt[3]) {
+    const int P = pr.P, I = c.I;
     const V step = (V)c.step;
-    if (THERMAL && thermal) {
-        // MathFunctions::unitNormalRand3 (MathFunctions.C:54-59)
-        V r1 = rng.u(1), r2 = rng.u(2), r3 = rng.u(3), r4 = rng.u(4);
-        V aux1 = sqrt((V)-2 * log(r1)), aux2 = (V)(2.0 * kPi) * r2;
-        V vers[3] = {aux1 * cos(aux2), aux1 * sin(aux2), sqrt((V)-2 * log(r3)) * cos((V)(2.0 * kPi) * r4)};
-        V randCS = e.nue * rng.u(5) / (V)c.N;
-        V prevSum = (V)0;
-        for (int g = 0; g < pr.G && chosen == -1; ++g) {
-            if (pr.gasFrac[g] == 0) continue;
-            int L = pr.gasFirst[g], R = pr.gasLast[g];
-            V sd = (V)pr.thermalStd[L];
-            for (int k = 0; k < 3; ++k) Vt[k] = vers[k] * sd;
-            V dx = e.vx - Vt[0], dy = e.vy - Vt[1], dz = e.vz - Vt[2];
-            V rel = sqrt(sqr_norm(dx, dy, dz));
-            V over = (V)0.5 * (V)pr.rMass[L] * rel * rel / (V)kCharge / step;
-            int k1 = (int)fmin(over, (V)(I - 1));
-            int k2 = (int)fmin((V)(k1 + 1), (V)(I - 1));
-            V w1 = (V)k2 - over;
-            w1 = w1 < (V)0 ? (V)0 : (V)1;
-            V w2 = (V)1 - w1;
-            const V* c1 = cumul + (size_t)k1 * P;
-            const V* c2 = cumul + (size_t)k2 * P;
-            const V* i1 = interp + (size_t)k1 * P;
-            const V* i2 = interp + (size_t)k2 * P;
-            V ref = (V)0;
-            if (L > 0) ref = lerp2(w1, c1[L - 1], w2, c2[L - 1]);
-            V lim = prevSum + (lerp2(w1, c1[R], w2, c2[R]) - ref) * rel;
-            if (randCS > lim) { prevSum = lim; continue; }
-            chosen = channel_search(c1, c2, i1, i2, w1, w2, L, R, randCS, prevSum, ref, rel, true);
-        }
-        if (chosen == -1) return;
-    } else {
-        V randF = e.nue * rng.u(5);
-        V over = e.eps / step;
+    // MathFunctions::unitNormalRand3 (MathFunctions.C:54-59)
+    const V r5 = rng.u(5);
+    V r1 = rng.u(1), r2 = rng.u(2), r3 = rng.u(3), r4 = rng.u(4);
+    V s2, c2r, s4, c4r;
+    turn_sincos(r2, s2, c2r);
+    turn_sincos(r4, s4, c4r);
+    V aux1 = sqrt((V)-2 * log_of(r1));
+    V vers[3] = {aux1 * c2r, aux1 * s2, sqrt((V)-2 * log_of(r3)) * c4r};
+    V randCS = e.nue * r5 / (V)c.N;
+    V prevSum = (V)0;
+    for (int g = 0; g < pr.G; ++g) {
+        if (pr.gasFrac[g] == 0) continue;
+        int L = pr.gasFirst[g], R = pr.gasLast[g];
+        V sd = (V)pr.thermalStd[L];
+        for (int k = 0; k < 3; ++k) Vt[k] = vers[k] * sd;
+        V dx = e.vx - Vt[0], dy = e.vy - Vt[1], dz = e.vz - Vt[2];
+        V rel = sqrt(sqr_norm(dx, dy, dz));
+        V over = (V)0.5 * (V)pr.rMass[L] * rel * rel / (V)kCharge / step;
         int k1 = (int)fmin(over, (V)(I - 1));
         int k2 = (int)fmin((V)(k1 + 1), (V)(I - 1));
         V w1 = (V)k2 - over;
-        if (w1 < (V)0) w1 = (V)0;
+        w1 = w1 < (V)0 ? (V)0 : (V)1;
         V w2 = (V)1 - w1;
-        V tot = lerp2(w1, nutot[k1], w2, nutot[k2]);
-        o.escape = e.nue < tot;
-        if (randF > tot) return;
-        const V* c1 = cumul + (size_t)k1 * P;
-        const V* c2 = cumul + (size_t)k2 * P;
+        const V* cr1 = cumul + (size_t)k1 * P;
+        const V* cr2 = cumul + (size_t)k2 * P;
         const V* i1 = interp + (size_t)k1 * P;
         const V* i2 = interp + (size_t)k2 * P;
-        V speed = sqrt(sqr_norm(e.vx, e.vy, e.vz));
-        V randCS = randF / (V)c.N / speed;
-        chosen = channel_search(c1, c2, i1, i2, w1, w2, 0, P - 1, randCS, (V)0, (V)0, speed, false);
+        V ref = (V)0;
+        if (L > 0) ref = lerp2(w1, cr1[L - 1], w2, cr2[L - 1]);
+        V lim = prevSum + (lerp2(w1, cr1[R], w2, cr2[R]) - ref) * rel;
+        if (randCS > lim) { prevSum = lim; continue; }
+        return channel_search(cr1, cr2, i1, i2, w1, w2, L, R, randCS, prevSum, ref, rel, true);
     }
+    return -1;
+}
+
+// The collision operator of process `chosen` (BoltzmannMC.C:1101-1280).
+template <typename V, bool THERMAL>
+__device__ __forceinline__ void apply_collision(const DevProcesses& pr, Electron<V>& e, EventRng<V>& rng,
+                                                CollOut<V>& o, int chosen, const V Vt[3], bool thermal) {
     o.pid = chosen;
     const int type = pr.type[chosen];
     const V inc = e.eps;
@@ -445,9 +533,10 @@ __device__ inline void collide(const CondParams& c, const DevProcesses& pr, cons
             if (relAfter <= 0) { o.pid = -1; return; }
             V cc = cos_chi<V>(pr, chosen, (V)relE, (V)relAfter, rng, 6);
             V sc = sqrt((V)1 - cc * cc);
-            V eta = (V)(2.0 * kPi) * rng.u(7);
+            V se, ce;
+            turn_sincos(rng.u(7), se, ce);
             V u[3];
-            euler(sc, cc, (V)sin(eta), (V)cos(eta), s, u);
+            euler(sc, cc, se, ce, s, u);
             double rel2 = (double)s.norm * (double)s.norm - 2.0 / mu * loss * kCharge;
             if (!(rel2 > 0)) { o.pid = -1; return; }  // rounding at the threshold edge
             V sp = (V)sqrt(rel2);
@@ -462,9 +551,10 @@ __device__ inline void collide(const CondParams& c, const DevProcesses& pr, cons
             if (after <= 0) { o.pid = -1; return; }
             V cc = cos_chi<V>(pr, chosen, inc, (V)after, rng, 6);
             V sc = sqrt((V)1 - cc * cc);
-            V eta = (V)(2.0 * kPi) * rng.u(7);
+            V se, ce;
+            turn_sincos(rng.u(7), se, ce);
             V u[3];
-            euler(sc, cc, (V)sin(eta), (V)cos(eta), s, u);
+            euler(sc, cc, se, ce, s, u);
             double v2after = (double)s.norm * (double)s.norm - 2.0 / kMass * loss * kCharge;
             if (!(v2after > 0)) { o.pid = -1; return; }  // rounding at the threshold edge
             V sp = sqrt((V)v2after * ((V)1 - (V)(2.0 * mu / (kMass + M)) * ((V)1 - cc)));
@@ -492,19 +582,16 @@ __device__ inline void collide(const CondParams& c, const DevProcesses& pr, cons
         V sCs, cCs, sCe, cCe, sEs, cEs, sEe, cEe;
         if (pr.scat[chosen] == 5) {
             cCs = sqrt(sc_e / net); sCs = sqrt((V)1 - cCs * cCs);
-            V eta = (V)(2.0 * kPi) * rng.u(7);
-            sEs = sin(eta); cEs = cos(eta);
+            turn_sincos(rng.u(7), sEs, cEs);
             cCe = sqrt(ej / net); sCe = sqrt((V)1 - cCe * cCe);
             sEe = -sEs; cEe = -cEs;
         } else {
             cCs = cos_chi<V>(pr, chosen, inc, sc_e, rng, 6);
             sCs = sqrt((V)1 - cCs * cCs);
-            V eta = (V)(2.0 * kPi) * rng.u(7);
-            sEs = sin(eta); cEs = cos(eta);
+            turn_sincos(rng.u(7), sEs, cEs);
             cCe = cos_chi<V>(pr, chosen, inc, ej, rng, 9);
             sCe = sqrt((V)1 - cCe * cCe);
-            V etaE = (V)(2.0 * kPi) * rng.u(10);
-            sEe = sin(etaE); cEe = cos(etaE);
+            turn_sincos(rng.u(10), sEe, cEe);
         }
         V u[3];
         euler(sCs, cCs, sEs, cEs, s, u);
@@ -519,18 +606,46 @@ __device__ inline void collide(const CondParams& c, const DevProcesses& pr, cons
     }
 }
 
-// One synchronization-interval step of an electron
-// (BoltzmannMC.C:637-680): draw a free time if none is pending, then fly
-// either to t_sync (partial flight, returns true = done) or to the collision.
-// Returns: 0 = collision happened (o filled), 1 = reached t_sync.
+template <typename V> __device__ __forceinline__ void clear_out(CollOut<V>& o) {
+    o.pid = -1; o.flags = 0; o.ejEps = (V)0; o.dE = 0; o.escape = false;
+    o.ejv[0] = o.ejv[1] = o.ejv[2] = (V)0;
+}
+
+// performCollision: null test, channel choice and collision operator.
+template <typename V, bool THERMAL>
+__device__ inline void collide(const CondParams& c, const DevProcesses& pr, const V* cumul, const V* interp,
+                               const V* nutot, Electron<V>& e, EventRng<V>& rng, CollOut<V>& o) {
+    clear_out(o);
+    V Vt[3] = {(V)0, (V)0, (V)0};
+    const bool thermal = THERMAL && is_thermal(c, pr, e);
+    int chosen;
+    if (THERMAL && thermal) {
+        chosen = thermal_choose(c, pr, cumul, interp, e, rng, Vt);
+        if (chosen == -1) return;
+    } else {
+        if (!cold_null_test(c, nutot, e, rng, o)) return;
+        chosen = cold_choose(c, pr, cumul, interp, e, rng);
+    }
+    apply_collision<V, THERMAL>(pr, e, rng, o, chosen, Vt, thermal);
+}
+
+// ------------------------------------------------------ two-phase step
+// One step of an electron in the passes of BoltzmannMC.C:637-680, split so
+// that a wave can run the real collision operator for many electrons at
+// once.  Phase A: draw a free time if none is pending (the first variate of
+// call 0 of the electron's current event), fly to t_sync or to the collision
+// and, in the cold branch, run the null test.  Phase B: the channel choice
+// and collision operator of a real (or thermal) collision.
+enum StepResult { STEP_DONE = 1, STEP_NULL = 2, STEP_REAL = 3 };
+
 template <typename V, int FC, bool THERMAL>
-__device__ __forceinline__ int interval_step(const CondParams& c, const DevProcesses& pr, const V* cumul,
-                                             const V* interp, const V* nutot, Electron<V>& e,
-                                             CollOut<V>& o, double& gain) {
-    if (e.t == c.tsync) return 1;
-    if (e.tfree < (V)0) {  // pending free time undefined (NON_DEF is negative)
-        EventRng<V> r0(c.seed, e.stream, e.event);
-        e.tfree = -log(r0.u(0)) / (V)c.trial;
+__device__ __forceinline__ int step_a(const CondParams& c, const DevProcesses& pr, const V* nutot, Electron<V>& e,
+                                      EventRng<V>& rng, CollOut<V>& o, double& gain) {
+    if (e.t == c.tsync) return STEP_DONE;
+    const bool pending = e.tfree < (V)0;  // pending free time undefined (NON_DEF is negative)
+    if (pending || e.t + (double)e.tfree <= c.tsync) rng.prime0();
+    if (pending) {
+        e.tfree = free_time(rng.u(0), c);
         e.nue = (V)c.trial;
     }
     if (e.t + (double)e.tfree > c.tsync) {
@@ -538,17 +653,41 @@ __device__ __forceinline__ int interval_step(const CondParams& c, const DevProce
         flight<FC>(c, e, dt, gain);
         e.t = c.tsync;
         e.tfree = (V)((double)e.tfree - dt);
-        return 1;
+        return STEP_DONE;
     }
     flight<FC>(c, e, (double)e.tfree, gain);
     e.t += (double)e.tfree;
-    EventRng<V> rng(c.seed, e.stream, e.event);
-    collide<V, THERMAL>(c, pr, cumul, interp, nutot, e, rng, o);
+    if (THERMAL && is_thermal(c, pr, e)) return STEP_REAL;
+    return cold_null_test(c, nutot, e, rng, o) ? STEP_REAL : STEP_NULL;
+}
+
+template <typename V, bool THERMAL>
+__device__ __forceinline__ void step_b(const CondParams& c, const DevProcesses& pr, const V* cumul, const V* interp,
+                                       Electron<V>& e, EventRng<V>& rng, CollOut<V>& o) {
+    clear_out(o);
+    V Vt[3] = {(V)0, (V)0, (V)0};
+    const bool thermal = THERMAL && is_thermal(c, pr, e);
+    int chosen;
+    if (THERMAL && thermal) {
+        chosen = thermal_choose(c, pr, cumul, interp, e, rng, Vt);
+        if (chosen == -1) return;
+    } else {
+        chosen = cold_choose(c, pr, cumul, interp, e, rng);
+    }
+    apply_collision<V, THERMAL>(pr, e, rng, o, chosen, Vt, thermal);
+}
+
+// After a collision event: the next event's free time is drawn when the next
+// step starts, except for a collision exactly at t_sync, whose next free
+// time is drawn now with this interval's trial frequency (BoltzmannMC.C:672).
+template <typename V> __device__ __forceinline__ void after_event(const CondParams& c, Electron<V>& e) {
     e.event += 1;
-    EventRng<V> rn(c.seed, e.stream, e.event);
-    e.tfree = -log(rn.u(0)) / (V)c.trial;
-    e.nue = (V)c.trial;
-    return 0;
+    e.tfree = (V)kNonDef;
+    if (e.t == c.tsync) {
+        EventRng<V> r(c.seed, e.stream, e.event);
+        e.tfree = free_time(r.u(0), c);
+        e.nue = (V)c.trial;
+    }
 }
 
 }  // namespace ekmc

@@ -753,6 +753,7 @@ public:
                 C.perBulkV.assign(3 * nPhases, 0.0); C.perFlux.assign(9 * nPhases, 0.0); C.perBulkD.assign(9 * nPhases, 0.0);
             }
             hcp[c].trial = 1.0;
+            hcp[c].trialInv = 1.0;
             hcp[c].tsync = 0;
         }
         HIP_TRY(hipMemcpyAsync(dconds.p, hcp.data(), nCond * sizeof(CondParams), hipMemcpyHostToDevice, stream));
@@ -808,6 +809,7 @@ public:
                 ++C.nSync;
                 ++C.syncIndex;
                 hcp[c].trial = C.trial;
+                hcp[c].trialInv = 1.0 / C.trial;
                 hcp[c].tsync = C.tsync;
                 hcp[c].step = C.tab.step;
                 hcp[c].syncIndex = C.syncIndex;

@@ -48,6 +48,9 @@ static inline void timer_stop(int k, hipStream_t st) {
 
 constexpr double kFix = 16777216.0;  // 2^24: fixed-point scale of energy sums [eV]
 constexpr int kWavesPerBlock = kBlock / kWave;
+// Lanes of a wave that wait for a real collision before the wave runs the
+// collision operator for all of them.
+constexpr int kParkMin = 16;
 
 // ------------------------------------------------------------ wave helpers
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
@@ -287,9 +290,10 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
     const int count = ch.count;
     int nextMain = 0, nq = 0, nextNew = 0, holes = 0;
     Electron<V> e;
-    bool have = false, isNew = false, holeOnLoad = false;
+    bool have = false, isNew = false, holeOnLoad = false, parked = false;
     int64_t slot = 0;
     uint8_t al = 1;
+    uint32_t c0[4] = {0, 0, 0, 0};  // call 0 of a parked electron's event
     uint32_t iter = 0, nNull = 0, nReal = 0, nEsc = 0, status = 0;
     double fieldGain = 0, maxE = 0;
     while (true) {
@@ -322,20 +326,23 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
             nextMain += takeMain;
             nextNew += takeNew;
         }
-        if (__ballot(have) == 0) break;
+        const uint64_t live = __ballot(have);
+        if (live == 0) break;
 
         bool done = false, dead = false, spawn = false;
         CollOut<V> o;
         o.pid = -1;
-        if (have) {
+        // phase A: flight, and the null test of the cold branch
+        if (have && !parked) {
             if (holeOnLoad) {
                 done = true;
                 dead = true;
             } else {
                 double gain = 0;
-                const int st = interval_step<V, FC, THERMAL>(cp, pr, cumul, interp, nutot, e, o, gain);
+                EventRng<V> rng(cp.seed, e.stream, e.event);
+                const int st = step_a<V, FC, THERMAL>(cp, pr, nutot, e, rng, o, gain);
                 fieldGain += gain;
-                if (st) {
+                if (st == STEP_DONE) {
                     done = true;
                 } else if (!isfinite(e.t) || !isfinite((double)e.eps)) {
                     status |= ST_NONFINITE;
@@ -343,21 +350,39 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
                     done = true;
                 } else {
                     if (o.escape) ++nEsc;
-                    status |= o.flags;
-                    if (o.pid < 0) {
+                    if (st == STEP_NULL) {
                         ++nNull;
+                        after_event(cp, e);
+                        if (++iter >= kMaxLoopPerSync) { status |= ST_LOOP_CAP; done = true; }
                     } else {
-                        ++nReal;
-                        atomicAdd(&cnt[o.pid], 1ull);
-                        const long long fx = __double2ll_rn(o.dE * kFix);
-                        if (o.dE >= 0) atomicAdd((unsigned long long*)&gainL[o.pid], (unsigned long long)fx);
-                        else atomicAdd((unsigned long long*)&lossL[o.pid], (unsigned long long)fx);
-                        const int type = pr.type[o.pid];
-                        if (type == 1) spawn = true;
-                        else if (type == 2) { done = true; dead = true; }
+                        parked = true;
+                        c0[0] = rng.w0[0]; c0[1] = rng.w0[1]; c0[2] = rng.w0[2]; c0[3] = rng.w0[3];
                     }
-                    if (++iter >= kMaxLoopPerSync) { status |= ST_LOOP_CAP; done = true; }
                 }
+            }
+        }
+        // phase B: real collisions, once enough lanes wait for one (or all do)
+        const uint64_t pk = __ballot(parked);
+        if (pk && (__popcll(pk) >= kParkMin || pk == (live & ~__ballot(done)))) {
+            if (parked) {
+                EventRng<V> rng(cp.seed, e.stream, e.event, c0);
+                step_b<V, THERMAL>(cp, pr, cumul, interp, e, rng, o);
+                status |= o.flags;
+                if (o.pid < 0) {
+                    ++nNull;
+                } else {
+                    ++nReal;
+                    atomicAdd(&cnt[o.pid], 1ull);
+                    const long long fx = __double2ll_rn(o.dE * kFix);
+                    if (o.dE >= 0) atomicAdd((unsigned long long*)&gainL[o.pid], (unsigned long long)fx);
+                    else atomicAdd((unsigned long long*)&lossL[o.pid], (unsigned long long)fx);
+                    const int type = pr.type[o.pid];
+                    if (type == 1) spawn = true;
+                    else if (type == 2) { done = true; dead = true; }
+                }
+                after_event(cp, e);
+                parked = false;
+                if (++iter >= kMaxLoopPerSync) { status |= ST_LOOP_CAP; done = true; }
             }
         }
         // ejected electrons join the chunk's newborn queue, in lane order

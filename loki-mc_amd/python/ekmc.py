@@ -24,7 +24,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 GASES = os.path.join(PKG, "gases")
-LIB_PATH = os.path.join(PKG, "lib", "libekmc.so")
+# EKMC_LIBRARY selects another build of the engine (e.g. a tuning variant)
+LIB_PATH = os.environ.get("EKMC_LIBRARY") or os.path.join(PKG, "lib", "libekmc.so")
 
 AMU = 1.660539040e-27          # Constant.h:9
 BOLTZMANN = 1.38064852e-23     # Constant.h:6

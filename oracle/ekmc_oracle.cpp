@@ -61,8 +61,10 @@ inline void philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], ui
 // Variate slots (ekmc.h) and their intervals:
 //   0: (0,1)   1-4: (0,1]   5: (0,1]   6: [0,1]   7: (0,1]
 //   8: [0,1]   9: [0,1]     10: (0,1]
-// A slot's 53-bit integer x comes from the two 32-bit words (2h, 2h+1) of
-// Philox call slot/2: x = ((w0 << 32) | w1) >> 11.
+// A slot's 53-bit integer x comes from two 32-bit words (2h, 2h+1) of one
+// Philox call of the event: x = ((w0 << 32) | w1) >> 11.  Slot -> (call, h):
+// call 0 {0,5}, call 1 {6,7}, call 2 {1,2}, call 3 {3,4}, call 4 {8,9},
+// call 5 {10}: the free time and the channel-choice variate share call 0.
 enum Interval { OPEN_OPEN, OPEN_CLOSED, CLOSED_CLOSED, CLOSED_OPEN };
 constexpr Interval kSlotInterval[EKMC_VARIATES_PER_EVENT] = {
     OPEN_OPEN, OPEN_CLOSED, OPEN_CLOSED, OPEN_CLOSED, OPEN_CLOSED, OPEN_CLOSED,
@@ -86,10 +88,13 @@ inline void philox_words(uint64_t seed, uint64_t stream, uint32_t event, uint32_
     philox4x32_10(ctr, key, out);
 }
 
+constexpr int kSlotCall[EKMC_VARIATES_PER_EVENT] = {0, 2, 2, 3, 3, 0, 1, 1, 4, 4, 5};
+constexpr int kSlotHalf[EKMC_VARIATES_PER_EVENT] = {0, 0, 1, 0, 1, 1, 0, 1, 0, 1, 0};
+
 inline double slot_variate(uint64_t seed, uint64_t stream, uint32_t event, int slot) {
     uint32_t w[4];
-    philox_words(seed, stream, event, P_EVENT, (uint32_t)(slot >> 1), w);
-    int h = slot & 1;
+    philox_words(seed, stream, event, P_EVENT, (uint32_t)kSlotCall[slot], w);
+    int h = kSlotHalf[slot];
     uint64_t x = ((((uint64_t)w[2 * h]) << 32) | w[2 * h + 1]) >> 11;
     return to_unit(x, kSlotInterval[slot]);
 }

@@ -24,6 +24,22 @@ def test_philox_known_answers():
     assert out.tolist() == [list(k[2]) for k in KAT]
 
 
+def test_variate_slots_map_to_documented_philox_words():
+    """FP64 slot -> (call, word pair) map of include/ekmc.h: call 0 holds the
+    free time (words 0,1) and the channel-choice variate (words 2,3)."""
+    seed, stream, ev = 0x0123456789ABCDEF, 0x00000005DEADBEEF, 77
+    calls = {0: (0, 0), 5: (0, 1), 6: (1, 0), 7: (1, 1), 1: (2, 0), 2: (2, 1), 3: (3, 0), 4: (3, 1),
+             8: (4, 0), 9: (4, 1), 10: (5, 0)}
+    key = np.array([[seed & 0xffffffff, seed >> 32]], np.uint32)
+    for slot, (call, h) in calls.items():
+        ctr = np.array([[stream & 0xffffffff, stream >> 32, ev, call]], np.uint32)
+        w = O.philox(ctr, key)[0].astype(np.uint64)
+        x = int(((w[2 * h] << np.uint64(32)) | w[2 * h + 1]) >> np.uint64(11))
+        k = 2.0 ** -53
+        want = (x + 0.5) * k if slot == 0 else ((x + 1.0) * k if slot in (1, 2, 3, 4, 5, 7, 10) else x * k)
+        assert O.variate(seed, stream, ev, slot) == want, slot
+
+
 def test_variate_slots_respect_their_intervals():
     vals = np.array([[O.variate(5, s, e, slot) for slot in range(11)]
                      for s in range(40) for e in range(25)])
