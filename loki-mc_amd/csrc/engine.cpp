@@ -54,6 +54,20 @@ template <typename T> struct DevBuf {
     void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
 };
 
+// Page-locked host buffer (hipHostMalloc) for the per-interval copies.
+template <typename T> struct PinnedBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count) {
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault);
+    }
+    void release() { if (p) (void)hipHostFree(p); p = nullptr; n = 0; }
+    T& operator[](size_t i) { return p[i]; }
+    const T& operator[](size_t i) const { return p[i]; }
+};
+
 std::vector<double> linspaced(int n, double a, double b) {
     // Eigen's LinSpaced(n, a, b): a + i*(b-a)/(n-1), the last point exactly b
     std::vector<double> out(n);
@@ -217,7 +231,9 @@ public:
     int summarySize = 0, recSize = 0;
     int nCells = 1000, nCos = 100, nRad = 200, nAx = 200, nPhases = 0;
     bool anyCyl = false, anyAC = false;
-    std::vector<double> hsum;
+    PinnedBuf<double> hsum;         // per-condition summaries of the last interval
+    PinnedBuf<CondParams> pinCond;  // staging of the per-interval condition upload
+    DevBuf<SlotRange> ranges;
     KernelTimer timer;
 
     ~Engine() override {
@@ -233,7 +249,8 @@ public:
         holeOff.release(); surOff.release(); markOff.release(); mainBase.release(); tailC.release();
         momC.release(); summary.release(); thermalDev.release(); grid.release(); accCount.release();
         accGain.release(); accLoss.release(); plans.release(); iproc.release(); dproc.release();
-        hEedf.release(); hPer.release(); hEadf.release(); hEvdf.release();
+        hEedf.release(); hPer.release(); hEadf.release(); hEvdf.release(); ranges.release();
+        hsum.release(); pinCond.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -446,7 +463,23 @@ public:
         HIP_TRY(thermalDev.alloc(n));
         HIP_TRY(hipMemcpyAsync(mainBase.p, mb.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, stream));
         HIP_TRY(hipMemcpyAsync(thermalDev.p, td.data(), n * sizeof(double), hipMemcpyHostToDevice, stream));
-        hsum.assign((size_t)n * summarySize, 0.0);
+        HIP_TRY(hsum.alloc((size_t)n * summarySize));
+        std::memset(hsum.p, 0, hsum.n * sizeof(double));
+        HIP_TRY(pinCond.alloc(n));
+        // histogram blocks: ranges of at most 8192 main slots of one condition
+        std::vector<SlotRange> hr;
+        for (int c = 0; c < n; ++c)
+            for (int64_t o = 0; o < hcp[c].nMain; o += 8192) {
+                SlotRange r;
+                r.cond = c;
+                r.first = hcp[c].base + o;
+                r.count = (int32_t)std::min<int64_t>(8192, hcp[c].nMain - o);
+                hr.push_back(r);
+            }
+        HIP_TRY(ranges.alloc(hr.size()));
+        HIP_TRY(hipMemcpy(ranges.p, hr.data(), hr.size() * sizeof(SlotRange), hipMemcpyHostToDevice));
+        dh.ranges = ranges.p;
+        dh.nRanges = (int)hr.size();
         HIP_TRY(hipStreamSynchronize(stream));
         return EKMC_OK;
     }
@@ -537,7 +570,7 @@ public:
 
     ekmc_status run_reduce_and_fetch() {
         HIP_TRY(launch_finish(dw, P, stream));
-        HIP_TRY(hipMemcpyAsync(hsum.data(), summary.p, hsum.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipMemcpyAsync(hsum.p, summary.p, hsum.n * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         sync_timers();
         return EKMC_OK;
@@ -817,19 +850,17 @@ public:
             if (active == 0 && !anyHist) break;
             if (anyHist) {
                 // histograms of the state sampled at the end of the previous interval
-                std::vector<CondParams> hc = hcp;
-                for (int c = 0; c < nCond; ++c) hc[c].active = 1;
-                HIP_TRY(hipMemcpyAsync(dconds.p, hc.data(), nCond * sizeof(CondParams), hipMemcpyHostToDevice, stream));
                 HIP_TRY(hipMemcpyAsync(grid.p, hgrid.data(), hgrid.size() * sizeof(double), hipMemcpyHostToDevice, stream));
                 HIP_TRY(hipMemcpyAsync(phaseRow.p, hrow.data(), nCond * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-                HIP_TRY(launch_hist<V>(ds, dw, dh, stream));
+                HIP_TRY(launch_hist<V>(ds, dh, stream));
             }
             if (active == 0) { HIP_TRY(hipStreamSynchronize(stream)); break; }
             {
                 ekmc_status st = rebuild_lists();
                 if (st != EKMC_OK) return st;
             }
-            HIP_TRY(hipMemcpyAsync(dconds.p, hcp.data(), nCond * sizeof(CondParams), hipMemcpyHostToDevice, stream));
+            std::memcpy(pinCond.p, hcp.data(), nCond * sizeof(CondParams));
+            HIP_TRY(hipMemcpyAsync(dconds.p, pinCond.p, nCond * sizeof(CondParams), hipMemcpyHostToDevice, stream));
             HIP_TRY(launch_interval<V>(ds, dw, dpr, tables.p, fl, stream));
             HIP_TRY(launch_sample<V>(ds, dw, stream));
             ekmc_status st = run_reduce_and_fetch();
@@ -930,12 +961,9 @@ public:
             if (C.histRequest) { gr[5] = 1; any = true; hrow[c] = C.histPhase; C.histRequest = 0; }
         }
         if (any) {
-            std::vector<CondParams> hc = hcp;
-            for (int c = 0; c < nCond; ++c) hc[c].active = 1;
-            HIP_TRY(hipMemcpyAsync(dconds.p, hc.data(), nCond * sizeof(CondParams), hipMemcpyHostToDevice, stream));
             HIP_TRY(hipMemcpyAsync(grid.p, hgrid.data(), hgrid.size() * sizeof(double), hipMemcpyHostToDevice, stream));
             HIP_TRY(hipMemcpyAsync(phaseRow.p, hrow.data(), nCond * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-            HIP_TRY(launch_hist<V>(ds, dw, dh, stream));
+            HIP_TRY(launch_hist<V>(ds, dh, stream));
         }
         HIP_TRY(hipStreamSynchronize(stream));
         sync_timers();

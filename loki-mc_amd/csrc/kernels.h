@@ -75,6 +75,13 @@ struct DevWork {
     int nCond;
 };
 
+// A contiguous range of main slots of one condition (histogram blocks).
+struct SlotRange {
+    int32_t cond;
+    int32_t count;
+    int64_t first;
+};
+
 // Histogram accumulators (integer counts summed over samples) and the grid of
 // each condition.  grid[6c..6c+6) = {eedf step, cos step, radial step, axial
 // step, max speed, fill flag}; phaseRow[c] is the AC phase row or -1.
@@ -86,6 +93,8 @@ struct DevHist {
     const double* grid;
     const int32_t* phaseRow;
     const int32_t* cyl;           // [n_cond] 1 = cylindrically symmetric
+    const SlotRange* ranges;      // [nRanges] main-slot ranges, one block each
+    int nRanges;
     int nCells, nCos, nRad, nAx, nPhases;
 };
 
@@ -106,7 +115,7 @@ hipError_t launch_interval(const DevState<V>& s, const DevWork& w, const DevProc
 template <typename V>
 hipError_t launch_sample(const DevState<V>& s, const DevWork& w, hipStream_t st);
 template <typename V>
-hipError_t launch_hist(const DevState<V>& s, const DevWork& w, const DevHist& h, hipStream_t st);
+hipError_t launch_hist(const DevState<V>& s, const DevHist& h, hipStream_t st);
 hipError_t launch_finish(const DevWork& w, int P, hipStream_t st);
 // Resident blocks per CU of the flight kernel of field case fc.
 template <typename V>

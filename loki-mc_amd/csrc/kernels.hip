@@ -50,7 +50,7 @@ constexpr double kFix = 16777216.0;  // 2^24: fixed-point scale of energy sums [
 constexpr int kWavesPerBlock = kBlock / kWave;
 // Lanes of a wave that wait for a real collision before the wave runs the
 // collision operator for all of them.
-constexpr int kParkMin = 16;
+constexpr int kParkMin = 32;
 
 // ------------------------------------------------------------ wave helpers
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
@@ -833,29 +833,27 @@ __global__ __launch_bounds__(kBlock) void k_sample(DevState<V> s, DevWork w) {
 
 // Histograms of one sampling instant (getTimeDependDistributions,
 // BoltzmannMC.C:1551-1571; MathFunctions::histogramCount/histogram2DCount,
-// MathFunctions.C:61-81,106-127), added to integer accumulators.  The EEDF is
-// counted per wave in LDS when it fits, else directly in HBM.
+// MathFunctions.C:61-81,106-127), added to integer accumulators.  One block
+// per range of main slots of one condition; the EEDF is counted in LDS when
+// it fits, else directly in HBM.
 template <typename V>
-__global__ __launch_bounds__(kBlock) void k_hist(DevState<V> s, DevWork w, DevHist h, int ldsCells) {
+__global__ __launch_bounds__(kBlock) void k_hist(DevState<V> s, const SlotRange* ranges, DevHist h, int ldsCells) {
     extern __shared__ unsigned char smem[];
-    const int lane = lane_id(), wid = threadIdx.x / kWave;
-    const int cid = blockIdx.x * kWavesPerBlock + wid;
-    if (cid >= w.nChunks) return;
-    const Chunk ch = w.chunks[cid];
-    const int c = ch.cond;
+    const SlotRange rg = ranges[blockIdx.x];
+    const int c = rg.cond;
     const double* g = h.grid + 6 * (size_t)c;
     if (g[5] == 0) return;
     const bool useLds = ldsCells >= h.nCells;
-    uint32_t* lhist = (uint32_t*)smem + (size_t)wid * ldsCells;
+    uint32_t* lhist = (uint32_t*)smem;
     if (useLds) {
-        for (int i = lane; i < h.nCells; i += kWave) lhist[i] = 0;
-        wave_lds_sync();
+        for (int i = threadIdx.x; i < h.nCells; i += kBlock) lhist[i] = 0;
+        __syncthreads();
     }
     const double eStep = g[0], cStep = g[1], rStep = g[2], aStep = g[3], maxSpeed = g[4];
     const bool cyl = h.cyl[c] != 0 && h.eadf;
     const int row = h.phaseRow[c];
-    for (int64_t k = lane; k < ch.count; k += kWave) {
-        const Rec<V> q = s.rec[ch.first + k];
+    for (int64_t k = threadIdx.x; k < rg.count; k += kBlock) {
+        const Rec<V> q = s.rec[rg.first + k];
         const double e = rec_energy(q);
         const int cell = (int)(e / eStep);
         if (cell >= 0 && cell < h.nCells) {
@@ -878,8 +876,8 @@ __global__ __launch_bounds__(kBlock) void k_hist(DevState<V> s, DevWork w, DevHi
         }
     }
     if (!useLds) return;
-    wave_lds_sync();
-    for (int i = lane; i < h.nCells; i += kWave) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < h.nCells; i += kBlock) {
         const uint32_t n = lhist[i];
         if (!n) continue;
         atomicAdd(&h.eedf[(size_t)c * h.nCells + i], (unsigned long long)n);
@@ -890,31 +888,47 @@ __global__ __launch_bounds__(kBlock) void k_hist(DevState<V> s, DevWork w, DevHi
 
 // ----------------------------------------------------------------- finish
 // summary[c] = [ event record (3P + 8) | moment sums (kMomentRecord) | plan (8) ]
-// Chunk records are summed in a fixed order: thread t takes chunks t, t+256,
-// ..., then the block sums the threads in order.  The integer accumulators
-// are converted and cleared for the next interval.  Block 0 resets the
-// flight-kernel work queues.
+// Chunk records are summed in a fixed order: thread t adds chunks t, t+256,
+// ... in order, the wave adds its lanes by butterflies, and thread f adds the
+// four waves in order.  The integer accumulators are converted and cleared
+// for the next interval.  Block 0 resets the flight-kernel work queues.
+constexpr int kFinishFields = CT_SIZE + kMomentRecord;
 __global__ __launch_bounds__(256) void k_finish(DevWork w, int P) {
-    __shared__ double red[256 / kWave];
+    __shared__ double red[kFinishFields][256 / kWave];
     const int c = blockIdx.x;
     if (c == 0 && threadIdx.x < 8) w.queue[threadIdx.x] = 0;
     const CondParams cp = w.conds[c];
     if (!cp.active) return;
     double* out = w.summary + (size_t)c * w.summarySize;
     const int R = w.recSize;
-    for (int f = 0; f < CT_SIZE + kMomentRecord; ++f) {
-        const bool isMax = f == CT_MAXE || f == CT_SIZE + kMomentRecord - 1;
-        double acc = 0;
-        for (int k = threadIdx.x; k < cp.nChunks; k += blockDim.x) {
-            const size_t j = (size_t)cp.chunkBegin + k;
-            const double v = f < CT_SIZE ? w.tailC[j * CT_SIZE + f] : w.momC[j * kMomentRecord + (f - CT_SIZE)];
-            acc = isMax ? fmax(acc, v) : acc + v;
-        }
-        const double tot = isMax ? block_max(acc, red) : block_sum(acc, red);
-        if (threadIdx.x == 0) {
-            if (f < CT_SIZE) out[3 * P + f] = tot;
-            else out[R + (f - CT_SIZE)] = tot;
-        }
+    double acc[kFinishFields];
+#pragma unroll
+    for (int f = 0; f < kFinishFields; ++f) acc[f] = 0;
+    for (int k = threadIdx.x; k < cp.nChunks; k += blockDim.x) {
+        const size_t j = (size_t)cp.chunkBegin + k;
+        const double* t = w.tailC + j * CT_SIZE;
+        const double* m = w.momC + j * kMomentRecord;
+#pragma unroll
+        for (int f = 0; f < CT_SIZE; ++f) acc[f] = f == CT_MAXE ? fmax(acc[f], t[f]) : acc[f] + t[f];
+#pragma unroll
+        for (int f = 0; f < kMomentRecord; ++f)
+            acc[CT_SIZE + f] = f == kMomentRecord - 1 ? fmax(acc[CT_SIZE + f], m[f]) : acc[CT_SIZE + f] + m[f];
+    }
+    const int lane = lane_id(), wid = threadIdx.x / kWave;
+#pragma unroll
+    for (int f = 0; f < kFinishFields; ++f) {
+        const bool isMax = f == CT_MAXE || f == kFinishFields - 1;
+        const double v = isMax ? wave_max(acc[f]) : wave_sum(acc[f]);
+        if (lane == 0) red[f][wid] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kFinishFields) {
+        const int f = threadIdx.x;
+        const bool isMax = f == CT_MAXE || f == kFinishFields - 1;
+        double tot = red[f][0];
+        for (int i = 1; i < 256 / kWave; ++i) tot = isMax ? fmax(tot, red[f][i]) : tot + red[f][i];
+        if (f < CT_SIZE) out[3 * P + f] = tot;
+        else out[R + (f - CT_SIZE)] = tot;
     }
     for (int p = threadIdx.x; p < P; p += blockDim.x) {
         const size_t i = (size_t)c * P + p;
@@ -1081,11 +1095,12 @@ hipError_t launch_sample(const DevState<V>& s, const DevWork& w, hipStream_t st)
 }
 
 template <typename V>
-hipError_t launch_hist(const DevState<V>& s, const DevWork& w, const DevHist& h, hipStream_t st) {
-    const int ldsCells = h.nCells <= 2048 ? h.nCells : 0;
-    const size_t lds = (size_t)kWavesPerBlock * ldsCells * 4;
+hipError_t launch_hist(const DevState<V>& s, const DevHist& h, hipStream_t st) {
+    if (h.nRanges == 0) return hipSuccess;
+    const int ldsCells = h.nCells <= 8192 ? h.nCells : 0;
+    const size_t lds = (size_t)ldsCells * 4;
     timer_start(K_HIST, st);
-    hipLaunchKernelGGL(k_hist<V>, dim3(chunk_grid(w.nChunks)), dim3(kBlock), lds, st, s, w, h, ldsCells);
+    hipLaunchKernelGGL(k_hist<V>, dim3(h.nRanges), dim3(kBlock), lds, st, s, h.ranges, h, ldsCells);
     timer_stop(K_HIST, st);
     return hipGetLastError();
 }
@@ -1128,7 +1143,7 @@ hipError_t launch_op_philox(int64_t n, const uint32_t* ctr, const uint32_t* key,
     template hipError_t launch_interval<V>(const DevState<V>&, const DevWork&, const DevProcesses&, const V*, \
                                            const FlightLists&, hipStream_t);                                  \
     template hipError_t launch_sample<V>(const DevState<V>&, const DevWork&, hipStream_t);                    \
-    template hipError_t launch_hist<V>(const DevState<V>&, const DevWork&, const DevHist&, hipStream_t);      \
+    template hipError_t launch_hist<V>(const DevState<V>&, const DevHist&, hipStream_t);                     \
     template int flight_blocks_per_cu<V>(int, bool, int);                                                     \
     template hipError_t launch_op_accelerate<V>(const CondParams*, int64_t, const double*, const double*,     \
                                                 double*, double*, double*, double*, hipStream_t);             \
