@@ -58,6 +58,15 @@ struct CondParams {
     double omega, cyclotron;
     int fieldCase;
     int I;                   // table rows
+    // FP32 copies of the field constants for the mixed-form flights
+    float fA[3];             // -e E / m_e
+    float fE0, fE2;          // E_x, E_z
+    float fW, fInvW;         // cyclotron frequency and its inverse
+    float fvEx;              // e E_x / (m_e W): E x B drift speed
+    float fAz;               // e E_z / m_e
+    float fEmw;              // e / (m_e omega)
+    float fInvOmega;         // 1 / omega
+    double freq;             // omega / (2 pi) [Hz]
     double N;                // gas density [m^-3]
     double gasEnergy;        // 1.5 k Tg / e [eV]
     // per interval

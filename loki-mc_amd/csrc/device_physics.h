@@ -261,7 +261,8 @@ __device__ inline void flight(const CondParams& c, Electron<double>& e, double d
 
 // Mixed form: FP32 velocities and displacements, FP64 clock and positions.
 // The DC, AC and DC+B cases use cancellation-free rewrites of the same
-// closed forms; the two AC+B cases run the FP64 formulas and round.
+// closed forms with per-condition FP32 constants; the two AC+B cases run the
+// FP64 formulas and round.
 template <int FC>
 __device__ inline void flight(const CondParams& c, Electron<float>& e, double dt, double& gain) {
     float prev = e.eps;
@@ -271,45 +272,47 @@ __device__ inline void flight(const CondParams& c, Electron<float>& e, double dt
     switch (fc) {
         case FIELD_DC: {
             float hdt2 = 0.5f * dtf * dtf;
-            float a0 = (float)c.accel[0], a1 = (float)c.accel[1], a2 = (float)c.accel[2];
-            e.x += (double)(v0 * dtf + a0 * hdt2);
-            e.y += (double)(v1 * dtf + a1 * hdt2);
-            e.z += (double)(v2 * dtf + a2 * hdt2);
-            e.vx = v0 + a0 * dtf;
-            e.vy = v1 + a1 * dtf;
-            e.vz = v2 + a2 * dtf;
+            e.x += (double)(v0 * dtf + c.fA[0] * hdt2);
+            e.y += (double)(v1 * dtf + c.fA[1] * hdt2);
+            e.z += (double)(v2 * dtf + c.fA[2] * hdt2);
+            e.vx = v0 + c.fA[0] * dtf;
+            e.vy = v1 + c.fA[1] * dtf;
+            e.vz = v2 + c.fA[2] * dtf;
         } break;
         case FIELD_AC: {
-            // phase from the FP64 clock, reduced to [0, 2 pi) before rounding
-            double phid = fmod(c.omega * e.t, 2.0 * kPi);
-            float phi = (float)phid, w = (float)c.omega, d = (float)(c.omega * dt);
-            float sinPhi = sinf(phi), cosPhi = cosf(phi);
-            float sh = sinf(0.5f * d), mid = phi + 0.5f * d;
-            float dS = 2.0f * cosf(mid) * sh;                       // sin(phi+d) - sin(phi)
-            float dmsin = -sin_minus_x(d);                            // d - sin(d)
-            float C1pd = sinPhi * dmsin - 2.0f * cosPhi * sh * sh;    // cos(phi+d)-cos(phi)+d sin(phi)
-            float e_me_w = (float)(kCharge / (kMass * c.omega));
-            float aux1 = e_me_w / w * C1pd;
-            float aux2 = -e_me_w * dS;
-            float E0 = (float)c.E[0], E2 = (float)c.E[2];
-            e.x += (double)(v0 * dtf + E0 * aux1);
+            // phase in turns from the FP64 clock (fraction of f t), then FP32
+            double ft = c.freq * e.t;
+            ft -= floor(ft);
+            const float phT = (float)ft, dT = (float)(c.freq * dt);
+            const float d = (float)(c.omega * dt);
+            const float sinPhi = __builtin_amdgcn_sinf(phT), cosPhi = __builtin_amdgcn_cosf(phT);
+            const float sh = sinf(0.5f * d);
+            const float dS = 2.0f * __builtin_amdgcn_cosf(phT + 0.5f * dT) * sh;  // sin(phi+d) - sin(phi)
+            const float dmsin = -sin_minus_x(d);                                   // d - sin(d)
+            const float C1pd = sinPhi * dmsin - 2.0f * cosPhi * sh * sh;           // cos(phi+d)-cos(phi)+d sin(phi)
+            const float aux1 = c.fEmw * c.fInvOmega * C1pd;
+            const float aux2 = -c.fEmw * dS;
+            e.x += (double)(v0 * dtf + c.fE0 * aux1);
             e.y += (double)(v1 * dtf);
-            e.z += (double)(v2 * dtf + E2 * aux1);
-            e.vx = v0 + E0 * aux2;
-            e.vz = v2 + E2 * aux2;
+            e.z += (double)(v2 * dtf + c.fE2 * aux1);
+            e.vx = v0 + c.fE0 * aux2;
+            e.vz = v2 + c.fE2 * aux2;
         } break;
         case FIELD_DC_B: {
-            float W = (float)c.cyclotron, x = (float)(c.cyclotron * dt);
-            float sW = sinf(x), cm1 = cos_minus_1(x);
-            float sW_W = sW / W, aux2 = cm1 / W;
-            float vEx = (float)(kCharge * c.E[0] / (kMass * c.cyclotron));
-            float az = (float)(kCharge * c.E[2] / kMass);
-            e.x += (double)(v0 * sW_W + (v1 + vEx) * aux2);
-            e.y += (double)(-v0 * aux2 + v1 * sW_W + vEx * (sin_minus_x(x) / W));
-            e.z += (double)(v2 * dtf - 0.5f * az * dtf * dtf);
+            // rotation by x = W dt from sin and cos of x/2:
+            // sin x = 2 s c, cos x - 1 = -2 s^2, sin x - x by series for small x
+            const float x = (float)(c.cyclotron * dt);
+            float sh, ch;
+            sincosf(0.5f * x, &sh, &ch);
+            const float sW = 2.0f * sh * ch, cm1 = -2.0f * sh * sh;
+            const float smx = fabsf(x) < 0.5f ? sin_minus_x(x) : sW - x;
+            const float vEx = c.fvEx, iW = c.fInvW;
+            e.x += (double)((v0 * sW + (v1 + vEx) * cm1) * iW);
+            e.y += (double)((-v0 * cm1 + v1 * sW + vEx * smx) * iW);
+            e.z += (double)(v2 * dtf - 0.5f * c.fAz * dtf * dtf);
             e.vx = v0 + v0 * cm1 - (v1 + vEx) * sW;
             e.vy = v0 * sW + v1 + (v1 + vEx) * cm1;
-            e.vz = v2 - az * dtf;
+            e.vz = v2 - c.fAz * dtf;
         } break;
         default: {
             Electron<double> d;

@@ -341,6 +341,16 @@ public:
             cp.omega = cs[c].omega;
             cp.cyclotron = cs[c].cyc;
             cp.fieldCase = cs[c].fieldCase;
+            for (int k = 0; k < 3; ++k) cp.fA[k] = (float)cs[c].accel[k];
+            cp.fE0 = (float)cs[c].E[0];
+            cp.fE2 = (float)cs[c].E[2];
+            cp.fW = (float)cs[c].cyc;
+            cp.fInvW = cs[c].cyc != 0 ? (float)(1.0 / cs[c].cyc) : 0.0f;
+            cp.fvEx = cs[c].cyc != 0 ? (float)(kCharge * cs[c].E[0] / (kMass * cs[c].cyc)) : 0.0f;
+            cp.fAz = (float)(kCharge * cs[c].E[2] / kMass);
+            cp.fEmw = cs[c].omega != 0 ? (float)(kCharge / (kMass * cs[c].omega)) : 0.0f;
+            cp.fInvOmega = cs[c].omega != 0 ? (float)(1.0 / cs[c].omega) : 0.0f;
+            cp.freq = cs[c].omega / (2.0 * kPi);
             cp.I = I;
             cp.N = cs[c].N;
             cp.gasEnergy = cs[c].gasEnergy;

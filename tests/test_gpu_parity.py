@@ -170,6 +170,40 @@ def _no_ionization_gas():
     return mixture(procs)
 
 
+@pytest.mark.parametrize("case", ["dc", "ac", "dcb", "acb", "ecr"])
+def test_mixed_flights_match_oracle_to_fp32_accuracy(case):
+    """The FP32 flights of the mixed form (cancellation-free rewrites for DC,
+    AC and DC+B; rounded FP64 formulas for AC+B and ECR) against the FP64
+    oracle: velocity change and displacement within FP32 accuracy."""
+    mix, _, num = ekmc.workload("cfg1_atomic40_dc")
+    num.precision = ekmc.MIXED
+    N = 3.22e22
+    W = ekmc.CHARGE * 800e-27 * N / ekmc.EMASS
+    cond = {"dc": condition(en=150.0, angle=60.0),
+            "ac": condition(en=150.0, angle=60.0, freq=3e9),
+            "dcb": condition(en=150.0, angle=90.0, bn=800.0),
+            "acb": condition(en=150.0, angle=45.0, freq=2e9, bn=800.0),
+            "ecr": condition(en=150.0, angle=45.0, freq=W / (2 * np.pi), bn=800.0)}[case]
+    cond.gas_density = N
+    cond.n_electrons = 256
+    eng = _engine(mix, [cond], num)
+    m = O.Model(mix, cond, num)
+    rng = np.random.default_rng(13)
+    n = 20000
+    t0 = rng.uniform(0, 5e-8, n)
+    dt = rng.uniform(1e-13, 3e-10, n)
+    v0 = (unit_vectors(rng, n) * speeds(rng.uniform(0.01, 40, n))[:, None]).astype(np.float32).astype(float)
+    r0 = rng.normal(size=(n, 3)) * 1e-2
+    pg, vg, eg, gg = eng.op_accelerate(0, t0, dt, r0, v0)
+    po, vo, eo, go = m.accelerate(t0, dt, r0, v0)
+    vscale = np.linalg.norm(v0, axis=1) + np.linalg.norm(vo - v0, axis=1)
+    assert np.all(np.linalg.norm(vg - vo, axis=1) <= 4e-6 * vscale)
+    dscale = np.linalg.norm(po - r0, axis=1) + 1e-30
+    assert np.all(np.linalg.norm((pg - r0) - (po - r0), axis=1) <= 4e-6 * dscale + 1e-18)
+    assert np.all(np.abs(eg - eo) <= 1e-5 * (eo + 1e-3))
+    eng.close()
+
+
 def test_initial_ensemble_and_one_interval_match_oracle():
     """Maxwellian start and one synchronization interval of 8192 electrons:
     same Philox streams, so every electron follows the same flights."""
