@@ -281,7 +281,10 @@ typedef struct {
 } ekmc_cond_status;
 ekmc_status ekmc_get_status(const ekmc_engine* engine, int32_t cond, ekmc_cond_status* out);
 
-/* Per-kernel HIP-event timings since creation (or the last reset). */
+/* Per-kernel HIP-event timings since creation (or the last reset), followed
+ * by host-side phases (names starting with "host_"): preparing and launching
+ * an interval, waiting for it, the per-condition control logic, table
+ * rebuilds and histogram downloads. */
 ekmc_status ekmc_get_kernel_stats(const ekmc_engine* engine, ekmc_kernel_stat* stats,
                                   int32_t* n_stats);
 ekmc_status ekmc_reset_kernel_stats(ekmc_engine* engine);
@@ -310,13 +313,21 @@ const char* ekmc_last_error(void);
  *   FP32: top 24 bits of one word; call 0 {0,5,6,7}, 1 {1,2,3,4}, 2 {8,9,10} */
 #define EKMC_VARIATES_PER_EVENT 11
 
-/* Collision tables of one condition as the engine built them
- * (interpolateCrossSections, BoltzmannMC.C:561-615) for a given maximum
- * energy: interp and cumul are [n_interp x n_processes] row-major. */
+/* Collision tables of one condition (interpolateCrossSections,
+ * BoltzmannMC.C:561-615) for a given maximum energy, built on the host in
+ * FP64: interp and cumul are [n_interp x n_processes] row-major.  Needs no
+ * device. */
 ekmc_status ekmc_build_tables(const ekmc_mixture* mixture, double gas_density,
                               int32_t n_interp, double max_energy,
                               double* interp, double* cumul, double* total_freq,
                               double* max_freq, double* energy_step);
+
+/* The same tables as the engine builds them on the device for condition
+ * `cond` (in the engine's precision, returned as FP64): rebuilds that
+ * condition's tables at max_energy and copies them back.  Any output pointer
+ * may be NULL. */
+ekmc_status ekmc_op_tables(ekmc_engine* engine, int32_t cond, double max_energy, double* interp,
+                           double* cumul, double* total_freq, double* max_freq, double* energy_step);
 
 /* Free flights (accelerateElectron, BoltzmannMC.C:804-905) on the device for
  * n electrons of condition `cond`: positions and velocities [3n] in/out,

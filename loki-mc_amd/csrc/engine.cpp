@@ -54,6 +54,14 @@ template <typename T> struct DevBuf {
     void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
 };
 
+// Host-side phases reported after the kernels by ekmc_get_kernel_stats.
+enum HostPhase { H_PREPARE = 0, H_WAIT, H_POST, H_TABLES, H_FLUSH, H_COUNT };
+const char* kHostNames[H_COUNT] = {"host_prepare", "host_wait", "host_post", "host_tables", "host_hist_flush"};
+using Clock = std::chrono::steady_clock;
+inline double ms_since(Clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+
 // Page-locked host buffer (hipHostMalloc) for the per-interval copies.
 template <typename T> struct PinnedBuf {
     T* p = nullptr;
@@ -109,7 +117,6 @@ struct Cond {
     // distributions
     double maxEedfE = 0, eedfStep = 0, cosStep = 0, radStep = 0, axStep = 0, maxSpeed = 0;
     std::vector<double> eedfNodes, eedfCells, eehSum, eehPer, eahSum, evhSum, cosCells, radCells, axCells;
-    bool gridDirty = false;
     int histRequest = 0;   // 1: histogram the current state before the next interval
     int histPhase = -1;
     int nPhases = 0;
@@ -137,6 +144,8 @@ public:
     virtual ekmc_status get_electrons(int c, double* pos, double* vel, double* e, double* t) = 0;
     virtual ekmc_status op_accelerate(int c, int64_t n, const double* t, const double* dt, double* pos,
                                       double* vel, double* e, double* g) = 0;
+    virtual ekmc_status op_tables(int c, double maxE, double* interp, double* cumul, double* tot, double* mx,
+                                  double* step) = 0;
     virtual ekmc_status op_collide(int c, double maxE, int64_t n, double* vel, double* e, const double* nue,
                                    const double* var, int32_t* pid, double* ejv, double* eje, double* de) = 0;
     ProcessList pl;
@@ -233,6 +242,10 @@ public:
     bool anyCyl = false, anyAC = false;
     PinnedBuf<double> hsum;         // per-condition summaries of the last interval
     PinnedBuf<CondParams> pinCond;  // staging of the per-interval condition upload
+    PinnedBuf<double> pinNumax;     // running maximum frequency of a rebuilt table
+    DevBuf<double> tsrcD, nutot64, numax64;
+    DevBuf<int> tsrcI;
+    DevTableSrc tsrc = {};
     DevBuf<SlotRange> ranges;
     KernelTimer timer;
 
@@ -250,7 +263,8 @@ public:
         momC.release(); summary.release(); thermalDev.release(); grid.release(); accCount.release();
         accGain.release(); accLoss.release(); plans.release(); iproc.release(); dproc.release();
         hEedf.release(); hPer.release(); hEadf.release(); hEvdf.release(); ranges.release();
-        hsum.release(); pinCond.release();
+        hsum.release(); pinCond.release(); pinNumax.release(); tsrcD.release(); tsrcI.release();
+        nutot64.release(); numax64.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -279,7 +293,7 @@ public:
             timer.armed[k] = false;
         }
         timer.enabled = true;
-        kstat.assign(K_COUNT, {0, 0.0});
+        kstat.assign(K_COUNT + H_COUNT, {0, 0.0});
         cs.assign(n, Cond());
         int64_t sumN = 0;
         for (int c = 0; c < n; ++c) {
@@ -440,6 +454,46 @@ public:
         dpr.type = iproc.p + oType; dpr.scat = iproc.p + oScat; dpr.gasFirst = iproc.p + oFirst; dpr.gasLast = iproc.p + oLast;
         dpr.params = dproc.p + oPar; dpr.energyLoss = dproc.p + oLoss; dpr.tMass = dproc.p + oM; dpr.rMass = dproc.p + oMu;
         dpr.thermalStd = dproc.p + oStd; dpr.wParam = dproc.p + oW; dpr.gasFrac = dproc.p + oFrac;
+        // collision-table sources: prepared cross sections and interpolation constants
+        {
+            std::vector<double> td, te, ts;
+            std::vector<int> ti(3 * P);
+            std::vector<int> off(P, 0);
+            for (int p = 0; p < P; ++p) {
+                if (pl.isSuper[p]) continue;
+                off[p] = (int)te.size();
+                te.insert(te.end(), pl.energy[p].begin(), pl.energy[p].end());
+                ts.insert(ts.end(), pl.sigma[p].begin(), pl.sigma[p].end());
+            }
+            for (int p = 0; p < P; ++p) {
+                const int q = pl.isSuper[p] ? p - 1 : p;
+                ti[p] = off[q];
+                ti[P + p] = off[q] + (int)pl.energy[q].size();
+                ti[2 * P + p] = pl.isSuper[p];
+            }
+            const size_t nPts = te.size();
+            td.reserve(2 * nPts + 5 * P);
+            td.insert(td.end(), te.begin(), te.end());
+            td.insert(td.end(), ts.begin(), ts.end());
+            for (int p = 0; p < P; ++p) td.push_back(pl.minE[p]);
+            for (int p = 0; p < P; ++p) td.push_back(pl.maxE[p]);
+            for (int p = 0; p < P; ++p) td.push_back(pl.relDens[p]);
+            for (int p = 0; p < P; ++p) td.push_back(pl.statW[p]);
+            for (int p = 0; p < P; ++p) td.push_back(pl.isSuper[p] ? pl.minE[p - 1] : 0.0);
+            HIP_TRY(tsrcD.alloc(td.size()));
+            HIP_TRY(tsrcI.alloc(ti.size()));
+            HIP_TRY(hipMemcpy(tsrcD.p, td.data(), td.size() * sizeof(double), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(tsrcI.p, ti.data(), ti.size() * sizeof(int), hipMemcpyHostToDevice));
+            tsrc.P = P; tsrc.I = I;
+            tsrc.energy = tsrcD.p; tsrc.sigma = tsrcD.p + nPts;
+            const double* pd = tsrcD.p + 2 * nPts;
+            tsrc.minE = pd; tsrc.maxE = pd + P; tsrc.relDens = pd + 2 * P; tsrc.statW = pd + 3 * P;
+            tsrc.thrPartner = pd + 4 * P;
+            tsrc.srcBegin = tsrcI.p; tsrc.srcEnd = tsrcI.p + P; tsrc.isSuper = tsrcI.p + 2 * P;
+            HIP_TRY(nutot64.alloc(I));
+            HIP_TRY(numax64.alloc(I));
+            HIP_TRY(pinNumax.alloc(I));
+        }
         // histograms
         HIP_TRY(hEedf.alloc((size_t)n * nCells));
         if (anyAC) HIP_TRY(hPer.alloc((size_t)n * nPhases * nCells));
@@ -495,28 +549,29 @@ public:
     }
 
     // ---------------------------------------------------- device helpers
-    ekmc_status upload_tables(int c) {
-        const Tables& tb = cs[c].tab;
-        std::vector<V> buf((size_t)I * (2 * P + 1));
-        size_t n = (size_t)I * P;
-        for (size_t i = 0; i < n; ++i) { buf[i] = (V)tb.cumul[i]; buf[n + i] = (V)tb.interp[i]; }
-        for (int i = 0; i < I; ++i) buf[2 * n + i] = (V)tb.nutot[i];
-        HIP_TRY(hipMemcpyAsync(tables.p + hcp[c].tableOffset, buf.data(), buf.size() * sizeof(V),
-                               hipMemcpyHostToDevice, stream));
+    // interpolateCrossSections (BoltzmannMC.C:561-615) on the device, into the
+    // condition's tables; the running maximum of the total frequency comes
+    // back for the trial-frequency check.  A rebuild to the same energy is
+    // skipped (BoltzmannMC.C:567-569).
+    ekmc_status interpolate(int c, double maxEnergy, bool force = false) {
+        Tables& tb = cs[c].tab;
+        if (maxEnergy == tb.maxEnergy && !force) return EKMC_OK;
+        const auto t0 = Clock::now();
+        tb.I = I;
+        tb.P = P;
+        tb.maxEnergy = maxEnergy;
+        tb.step = maxEnergy / (double)(I - 1);
+        HIP_TRY(launch_tables<V>(tsrc, cs[c].N, tb.step, tables.p + hcp[c].tableOffset, nutot64.p, numax64.p, stream));
+        HIP_TRY(hipMemcpyAsync(pinNumax.p, numax64.p, (size_t)I * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
+        tb.numax.assign(pinNumax.p, pinNumax.p + I);
+        host_add(H_TABLES, ms_since(t0));
         return EKMC_OK;
-    }
-
-    void interpolate(int c, double maxEnergy) {
-        // interpolateCrossSections skips a rebuild to the same energy (BoltzmannMC.C:567-569)
-        if (maxEnergy == cs[c].tab.maxEnergy) return;
-        build_tables(pl, cs[c].N, I, maxEnergy, cs[c].tab);
-        cs[c].gridDirty = true;
     }
 
     // checkMaxCollisionFrequency (BoltzmannMC.C:716-763) with the ensemble
     // maximum energy at the start of the interval.
-    void check_max_collision_frequency(int c) {
+    ekmc_status check_max_collision_frequency(int c) {
         Cond& C = cs[c];
         double thermalC = 0;
         if (num.gas_temperature_effect == EKMC_GASTEMP_TRUE || num.gas_temperature_effect == EKMC_GASTEMP_SMART)
@@ -526,12 +581,19 @@ public:
             updated = false;
             double maxEnergy = max_accel_energy(C, C.lastMaxE, 10.0 / C.trial) + thermalC;
             if (maxEnergy > C.tab.maxEnergy || 2.5 * maxEnergy < C.tab.maxEnergy) {
-                if (2.0 * maxEnergy < pl.energyMaxElastic) interpolate(c, 2.0 * maxEnergy);
-                else interpolate(c, pl.energyMaxElastic);
+                ekmc_status st = interpolate(c, 2.0 * maxEnergy < pl.energyMaxElastic ? 2.0 * maxEnergy
+                                                                                    : pl.energyMaxElastic);
+                if (st != EKMC_OK) return st;
             }
             int k = (int)std::fmin(std::ceil(maxEnergy / C.tab.step), I - 1);
             if (C.trial < C.tab.numax[k]) { updated = true; C.trial *= 1.1; }
         }
+        return EKMC_OK;
+    }
+
+    void host_add(int ph, double ms) {
+        kstat[K_COUNT + ph].first += 1;
+        kstat[K_COUNT + ph].second += ms;
     }
 
     ekmc_status sync_timers() {
@@ -581,7 +643,9 @@ public:
     ekmc_status run_reduce_and_fetch() {
         HIP_TRY(launch_finish(dw, P, stream));
         HIP_TRY(hipMemcpyAsync(hsum.p, summary.p, hsum.n * sizeof(double), hipMemcpyDeviceToHost, stream));
+        const auto t0 = Clock::now();
         HIP_TRY(hipStreamSynchronize(stream));
+        host_add(H_WAIT, ms_since(t0));
         sync_timers();
         return EKMC_OK;
     }
@@ -628,6 +692,8 @@ public:
     // Download the device histogram sums of condition c into the FP64 host
     // sums (with the grid they were filled on) and zero them on the device.
     ekmc_status flush_hist(int c) {
+        const auto t0 = Clock::now();
+        struct Done { Engine* e; Clock::time_point t; ~Done() { e->host_add(H_FLUSH, ms_since(t)); } } done{this, t0};
         Cond& C = cs[c];
         std::vector<unsigned long long> tmp;
         auto pull = [&](DevBuf<unsigned long long>& b, size_t off, size_t n, std::vector<double>& dst) -> ekmc_status {
@@ -810,10 +876,8 @@ public:
             const double* m = &hsum[(size_t)c * summarySize + recSize];
             C.lastMaxE = m[26];
             C.maxElecEnergy = m[26];
-            build_tables(pl, C.N, I, 2.0 * C.lastMaxE, C.tab);
+            if ((st = interpolate(c, 2.0 * C.lastMaxE)) != EKMC_OK) return st;
             C.trial = C.tab.numax[I - 1];
-            if ((st = upload_tables(c)) != EKMC_OK) return st;
-            C.gridDirty = false;
             C.nSamp = 1; C.nSync = 1;
             C.time = 0;
             calc_mean(C, m);
@@ -829,6 +893,7 @@ public:
         if (!initialized) return fail(EKMC_ERR_STATE, "ekmc_initialize has not been called");
         set_kernel_timer(&timer);
         for (int s = 0; s < nsync; ++s) {
+            const auto tPrep = Clock::now();
             int active = 0;
             bool anyHist = false;
             std::vector<double> hgrid((size_t)nCond * 6, 0.0);
@@ -841,11 +906,9 @@ public:
                 if (C.histRequest) { gr[5] = 1; anyHist = true; hrow[c] = C.histPhase; C.histRequest = 0; }
                 if (C.finished) continue;
                 ++active;
-                check_max_collision_frequency(c);
-                if (C.gridDirty) {
-                    ekmc_status st = upload_tables(c);
+                {
+                    ekmc_status st = check_max_collision_frequency(c);
                     if (st != EKMC_OK) return st;
-                    C.gridDirty = false;
                 }
                 double dts = num.sync_time_x_max_coll_freq / C.trial;
                 C.tsync = C.time + dts;
@@ -873,13 +936,16 @@ public:
             HIP_TRY(hipMemcpyAsync(dconds.p, pinCond.p, nCond * sizeof(CondParams), hipMemcpyHostToDevice, stream));
             HIP_TRY(launch_interval<V>(ds, dw, dpr, tables.p, fl, stream));
             HIP_TRY(launch_sample<V>(ds, dw, stream));
+            host_add(H_PREPARE, ms_since(tPrep));
             ekmc_status st = run_reduce_and_fetch();
             if (st != EKMC_OK) return st;
+            const auto tPost = Clock::now();
             for (int c = 0; c < nCond; ++c) {
                 if (cs[c].finished) continue;
                 st = post_interval(c);
                 if (st != EKMC_OK) return st;
             }
+            host_add(H_POST, ms_since(tPost));
         }
         int run = 0;
         for (auto& C : cs) run += C.finished ? 0 : 1;
@@ -1031,11 +1097,30 @@ public:
         return EKMC_OK;
     }
 
+    ekmc_status op_tables(int c, double maxE, double* interp, double* cumul, double* tot, double* mx,
+                          double* step) override {
+        if (!(maxE > 0)) return fail(EKMC_ERR_INVALID_ARGUMENT, "max_energy must be positive");
+        ekmc_status st = interpolate(c, maxE, true);
+        if (st != EKMC_OK) return st;
+        const size_t n = (size_t)I * P;
+        std::vector<V> h(2 * n + I);
+        HIP_TRY(hipMemcpy(h.data(), tables.p + hcp[c].tableOffset, h.size() * sizeof(V), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            if (cumul) cumul[i] = (double)h[i];
+            if (interp) interp[i] = (double)h[n + i];
+        }
+        for (int i = 0; i < I; ++i) {
+            if (tot) tot[i] = (double)h[2 * n + i];
+            if (mx) mx[i] = cs[c].tab.numax[i];
+        }
+        if (step) *step = cs[c].tab.step;
+        return EKMC_OK;
+    }
+
     ekmc_status op_collide(int c, double maxE, int64_t n, double* vel, double* e, const double* nuev,
                            const double* var, int32_t* pid, double* ejv, double* eje, double* de) override {
         if (maxE > 0) {
-            build_tables(pl, cs[c].N, I, maxE, cs[c].tab);
-            ekmc_status st = upload_tables(c);
+            ekmc_status st = interpolate(c, maxE);
             if (st != EKMC_OK) return st;
         } else if (cs[c].tab.maxEnergy <= 0) {
             return fail(EKMC_ERR_STATE, "no tables built for this condition; pass max_energy > 0");
@@ -1362,14 +1447,15 @@ ekmc_status ekmc_get_status(const ekmc_engine* e, int32_t cond, ekmc_cond_status
 
 ekmc_status ekmc_get_kernel_stats(const ekmc_engine* e, ekmc_kernel_stat* stats, int32_t* n_stats) {
     if (!e || !n_stats) return fail(EKMC_ERR_INVALID_ARGUMENT, "bad argument");
-    int n = std::min<int>(*n_stats, K_COUNT);
+    const int total = K_COUNT + H_COUNT;
+    int n = std::min<int>(*n_stats, total);
     for (int k = 0; k < n && stats; ++k) {
         std::memset(&stats[k], 0, sizeof(stats[k]));
-        std::snprintf(stats[k].name, sizeof(stats[k].name), "%s", kKernelNames[k]);
+        std::snprintf(stats[k].name, sizeof(stats[k].name), "%s", k < K_COUNT ? kKernelNames[k] : kHostNames[k - K_COUNT]);
         stats[k].launches = e->impl->kstat[k].first;
         stats[k].total_ms = e->impl->kstat[k].second;
     }
-    *n_stats = K_COUNT;
+    *n_stats = total;
     return EKMC_OK;
 }
 
@@ -1418,6 +1504,12 @@ ekmc_status ekmc_op_collide(ekmc_engine* e, int32_t cond, double max_energy, int
     if (n == 0) return EKMC_OK;
     return e->impl->op_collide(cond, max_energy, n, vel, energy, trial_freq, variates, process, ejected_vel,
                                ejected_energy, energy_change);
+}
+
+ekmc_status ekmc_op_tables(ekmc_engine* e, int32_t cond, double max_energy, double* interp, double* cumul,
+                           double* total_freq, double* max_freq, double* energy_step) {
+    if (!e || cond < 0 || cond >= (int)e->impl->cs.size()) return fail(EKMC_ERR_INVALID_ARGUMENT, "bad argument");
+    return e->impl->op_tables(cond, max_energy, interp, cumul, total_freq, max_freq, energy_step);
 }
 
 ekmc_status ekmc_op_philox(int64_t n, const uint32_t* counter, const uint32_t* key, uint32_t* out) {

@@ -105,6 +105,31 @@ struct FlightLists {
     int32_t grid[5];              // persistent blocks per launch
 };
 
+// Sources of the collision tables (interpolateCrossSections,
+// BoltzmannMC.C:561-615): each process's prepared integral cross section
+// (a superelastic process reads its inelastic partner's) and the constants of
+// the interpolation.
+struct DevTableSrc {
+    int P, I;
+    const double* energy;      // concatenated prepared tables
+    const double* sigma;
+    const int* srcBegin;       // [P] first point of the table process p reads
+    const int* srcEnd;         // [P] one past its last point
+    const int* isSuper;        // [P]
+    const double* minE;        // [P]
+    const double* maxE;        // [P]
+    const double* relDens;     // [P]
+    const double* statW;       // [P]
+    const double* thrPartner;  // [P] threshold of the inelastic partner (superelastic only)
+};
+
+// Build condition tables on the device for gas density N and energy step
+// `step` (= max energy / (I-1)): `tables` receives cumul [I*P], interp [I*P],
+// nutot [I] in V; nutot64 [I] and numax64 [I] (running maximum) in FP64.
+template <typename V>
+hipError_t launch_tables(const DevTableSrc& src, double N, double step, V* tables, double* nutot64,
+                         double* numax64, hipStream_t st);
+
 // mainBase[c]: index of condition c's first electron among all main electrons
 template <typename V>
 hipError_t launch_init(const DevState<V>& s, const DevWork& w, int64_t totalMain,

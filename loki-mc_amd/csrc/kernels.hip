@@ -47,6 +47,7 @@ static inline void timer_stop(int k, hipStream_t st) {
 }
 
 constexpr double kFix = 16777216.0;  // 2^24: fixed-point scale of energy sums [eV]
+static inline unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 constexpr int kWavesPerBlock = kBlock / kWave;
 // Lanes of a wave that wait for a real collision before the wave runs the
 // collision operator for all of them.
@@ -949,6 +950,88 @@ __global__ __launch_bounds__(256) void k_finish(DevWork w, int P) {
     }
 }
 
+// ----------------------------------------------------------------- tables
+// Linear interpolation as gsl_interp_linear evaluates it: the interval is the
+// last one with x[j] <= v (the last point belongs to the last interval), and
+// y = y_j + (v - x_j)/dx*(y_{j+1} - y_j).
+__device__ __forceinline__ double table_at(const double* x, const double* y, int n, double v) {
+#pragma clang fp contract(off)
+    int lo = 0, hi = n - 2;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (x[mid] <= v) lo = mid; else hi = mid - 1;
+    }
+    const double dx = x[lo + 1] - x[lo];
+    return dx > 0.0 ? y[lo] + (v - x[lo]) / dx * (y[lo + 1] - y[lo]) : 0.0;
+}
+
+// One table row per thread, in the arithmetic and order of the host FP64
+// build (interpolateCrossSections, BoltzmannMC.C:571-614).
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_table_rows(DevTableSrc a, double N, double step, V* tables,
+                                                       double* nutot64) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.I) return;
+    const int P = a.P;
+    V* cumul = tables;
+    V* interp = tables + (size_t)a.I * P;
+    V* nutot = interp + (size_t)a.I * P;
+    const double energy = i * step;
+    double acc = 0;
+    for (int p = 0; p < P; ++p) {
+        const int b = a.srcBegin[p], n = a.srcEnd[p] - b;
+        double value = 0;
+        if (a.isSuper[p]) {
+            // Klein-Rosseland: (g_i/g_f)(1 + E_th/u) sigma_if(u + E_th)
+            if (energy > a.minE[p] && energy <= a.maxE[p]) {
+                const double thr = a.thrPartner[p];
+                value = a.statW[p] * (1.0 + thr / energy) * table_at(a.energy + b, a.sigma + b, n, energy + thr) *
+                        a.relDens[p];
+            }
+        } else if (energy >= a.minE[p] && energy <= a.maxE[p]) {
+            value = table_at(a.energy + b, a.sigma + b, n, energy) * a.relDens[p];
+        }
+        interp[(size_t)i * P + p] = (V)value;
+        acc += value;
+        cumul[(size_t)i * P + p] = (V)acc;
+    }
+    acc *= N * sqrt(energy * 2.0 * kCharge / kMass);
+    nutot[i] = (V)acc;
+    nutot64[i] = acc;
+}
+
+// Running maximum of the total collision frequency (BoltzmannMC.C:612-613):
+// one block; each thread takes a contiguous segment.
+__global__ __launch_bounds__(1024) void k_table_max(const double* nutot64, double* numax64, int I) {
+    __shared__ double lds[1024];
+    const int per = (I + 1023) / 1024;
+    const int b = threadIdx.x * per, e = min(I, b + per);
+    double m = 0;
+    for (int i = b; i < e; ++i) m = fmax(m, nutot64[i]);
+    lds[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {  // inclusive max-scan of the segment maxima
+        const double v = threadIdx.x >= o ? lds[threadIdx.x - o] : 0.0;
+        __syncthreads();
+        lds[threadIdx.x] = fmax(lds[threadIdx.x], v);
+        __syncthreads();
+    }
+    double run = threadIdx.x > 0 ? lds[threadIdx.x - 1] : 0.0;
+    for (int i = b; i < e; ++i) {
+        run = fmax(nutot64[i], run);
+        numax64[i] = run;
+    }
+}
+
+template <typename V>
+hipError_t launch_tables(const DevTableSrc& src, double N, double step, V* tables, double* nutot64,
+                         double* numax64, hipStream_t st) {
+    hipLaunchKernelGGL(k_table_rows<V>, dim3(grid_of(src.I)), dim3(kBlock), 0, st, src, N, step, tables, nutot64);
+    hipLaunchKernelGGL(k_table_max, dim3(1), dim3(1024), 0, st, nutot64, numax64, src.I);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------- batch operators
 template <typename V>
 __global__ __launch_bounds__(kBlock) void k_op_accelerate(const CondParams* cond, int64_t n, const double* time,
@@ -1007,7 +1090,6 @@ __global__ __launch_bounds__(kBlock) void k_op_philox(int64_t n, const uint32_t*
 }
 
 // ------------------------------------------------------------- launchers
-static inline unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 static inline unsigned chunk_grid(int nChunks) { return (unsigned)((nChunks + kWavesPerBlock - 1) / kWavesPerBlock); }
 static inline size_t flight_lds(int P) { return (size_t)kWavesPerBlock * 3 * P * 8; }
 
@@ -1145,6 +1227,7 @@ hipError_t launch_op_philox(int64_t n, const uint32_t* ctr, const uint32_t* key,
     template hipError_t launch_sample<V>(const DevState<V>&, const DevWork&, hipStream_t);                    \
     template hipError_t launch_hist<V>(const DevState<V>&, const DevHist&, hipStream_t);                     \
     template int flight_blocks_per_cu<V>(int, bool, int);                                                     \
+    template hipError_t launch_tables<V>(const DevTableSrc&, double, double, V*, double*, double*, hipStream_t); \
     template hipError_t launch_op_accelerate<V>(const CondParams*, int64_t, const double*, const double*,     \
                                                 double*, double*, double*, double*, hipStream_t);             \
     template hipError_t launch_op_collide<V>(const CondParams*, const DevProcesses&, const V*, int64_t,       \

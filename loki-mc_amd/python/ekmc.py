@@ -385,6 +385,7 @@ def lib():
         L.ekmc_op_accelerate.argtypes = [C.c_void_p, I32, I64, pD, pD, pD, pD, pD, pD]
         L.ekmc_op_collide.argtypes = [C.c_void_p, I32, D, I64, pD, pD, pD, pD, pI32, pD, pD, pD]
         L.ekmc_op_philox.argtypes = [I64, pU32, pU32, pU32]
+        L.ekmc_op_tables.argtypes = [C.c_void_p, I32, D, pD, pD, pD, pD, pD]
         _lib = L
     return _lib
 
@@ -505,6 +506,15 @@ class Engine:
                                        dptr(np.ascontiguousarray(dt, np.float64)), dptr(pos),
                                        dptr(vel), dptr(energy), dptr(gain)))
         return pos, vel, energy, gain
+
+    def op_tables(self, cond, max_energy):
+        """Tables of `cond` rebuilt on the device at max_energy (engine precision)."""
+        P, I = self.mix.n_processes, self.numerics.n_interp_points
+        interp, cumul = np.zeros(I * P), np.zeros(I * P)
+        tot, mx, step = np.zeros(I), np.zeros(I), C.c_double()
+        check(lib().ekmc_op_tables(self.h, cond, max_energy, dptr(interp), dptr(cumul), dptr(tot),
+                                   dptr(mx), C.byref(step)))
+        return interp.reshape(I, P), cumul.reshape(I, P), tot, mx, step.value
 
     def op_collide(self, cond, max_energy, vel, energy, trial_freq, variates):
         n = len(energy)

@@ -204,6 +204,29 @@ def test_mixed_flights_match_oracle_to_fp32_accuracy(case):
     eng.close()
 
 
+@pytest.mark.parametrize("wl,prec", [("cfg3_o2like_batched", ekmc.FP64), ("cfg5_mix5_large", ekmc.FP64),
+                                     ("cfg3_o2like_batched", ekmc.MIXED)])
+def test_device_tables_match_host_build(wl, prec):
+    """Tables built on the device (every rebuild of a run) equal the host FP64
+    build, which the oracle pins: bit for bit in FP64, rounded to FP32 in the
+    mixed form."""
+    mix, conds, num = ekmc.workload(wl, n_electrons=1024, conditions=[0])
+    num.precision = prec
+    eng = _engine(mix, conds, num)
+    for emax in (0.37, 12.5, 97.0):
+        di, dc, dt, dm, ds = eng.op_tables(0, emax)
+        hi, hc, ht, hm, hs = ekmc.build_tables(mix, conds[0].gas_density, num.n_interp_points, emax)
+        assert ds == hs
+        if prec == ekmc.FP64:
+            for a, b in ((di, hi), (dc, hc), (dt, ht), (dm, hm)):
+                assert np.array_equal(a, b)
+        else:
+            for a, b in ((di, hi), (dc, hc), (dt, ht)):
+                assert np.array_equal(a, b.astype(np.float32).astype(float))
+            assert np.array_equal(dm, hm)
+    eng.close()
+
+
 def test_initial_ensemble_and_one_interval_match_oracle():
     """Maxwellian start and one synchronization interval of 8192 electrons:
     same Philox streams, so every electron follows the same flights."""
