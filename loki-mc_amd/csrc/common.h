@@ -15,6 +15,7 @@ constexpr int kBlock = 256;            // threads per block of every grid-wide k
 constexpr int kWave = 64;              // CDNA wavefront
 constexpr int kChunkDefault = 1024;    // main electrons per wave work unit (64 lanes x 16)
 constexpr uint32_t kMaxLoopPerSync = 1u << 22;  // flight cap per electron per interval
+constexpr int kEnvBins = 256;          // bins of the null-test envelope of the total frequency
 
 // Field cases of accelerateElectron (BoltzmannMC.C:809-897)
 enum FieldCase : int { FIELD_DC = 0, FIELD_AC = 1, FIELD_DC_B = 2, FIELD_ECR = 3, FIELD_AC_B = 4 };
@@ -58,6 +59,7 @@ struct CondParams {
     double omega, cyclotron;
     int fieldCase;
     int I;                   // table rows
+    int envShift;            // rows per null-test envelope bin = 1 << envShift
     // FP32 copies of the field constants for the mixed-form flights
     float fA[3];             // -e E / m_e
     float fE0, fE2;          // E_x, E_z

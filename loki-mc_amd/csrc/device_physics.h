@@ -455,6 +455,27 @@ __device__ __forceinline__ bool cold_null_test(const CondParams& c, const V* nut
     return !(randF > tot);
 }
 
+// The same decision with the per-bin envelope envHi/envLo of the total
+// frequency (bounds of every interpolated value a bin's rows can give,
+// widened by a few ulp): only a variate between the bounds reads the table.
+// The result, and the escape flag, equal those of cold_null_test.
+template <typename V>
+__device__ __forceinline__ bool cold_null_test_env(const CondParams& c, const V* nutot, const V* envHi,
+                                                   const V* envLo, const Electron<V>& e, EventRng<V>& rng,
+                                                   CollOut<V>& o) {
+    const ColdRow<V> r = cold_row(c, e);
+    const V randF = e.nue * rng.u(5);
+    const int bin = r.k1 >> c.envShift;
+    const V hi = envHi[bin], lo = envLo[bin];
+    const bool needTot = (randF > lo && !(randF > hi)) || e.nue < hi;
+    V tot = hi;
+    if (needTot) tot = lerp2(r.w1, nutot[r.k1], r.w2, nutot[r.k2]);
+    o.escape = needTot ? e.nue < tot : false;
+    if (randF > hi) return false;
+    if (!(randF > lo)) return true;
+    return !(randF > tot);
+}
+
 // Cold channel choice of a real collision (BoltzmannMC.C:1055-1097)
 template <typename V>
 __device__ __forceinline__ int cold_choose(const CondParams& c, const DevProcesses& pr, const V* cumul,
@@ -642,8 +663,9 @@ __device__ inline void collide(const CondParams& c, const DevProcesses& pr, cons
 enum StepResult { STEP_DONE = 1, STEP_NULL = 2, STEP_REAL = 3 };
 
 template <typename V, int FC, bool THERMAL>
-__device__ __forceinline__ int step_a(const CondParams& c, const DevProcesses& pr, const V* nutot, Electron<V>& e,
-                                      EventRng<V>& rng, CollOut<V>& o, double& gain) {
+__device__ __forceinline__ int step_a(const CondParams& c, const DevProcesses& pr, const V* nutot, const V* envHi,
+                                      const V* envLo, Electron<V>& e, EventRng<V>& rng, CollOut<V>& o,
+                                      double& gain) {
     if (e.t == c.tsync) return STEP_DONE;
     const bool pending = e.tfree < (V)0;  // pending free time undefined (NON_DEF is negative)
     if (pending || e.t + (double)e.tfree <= c.tsync) rng.prime0();
@@ -661,7 +683,7 @@ __device__ __forceinline__ int step_a(const CondParams& c, const DevProcesses& p
     flight<FC>(c, e, (double)e.tfree, gain);
     e.t += (double)e.tfree;
     if (THERMAL && is_thermal(c, pr, e)) return STEP_REAL;
-    return cold_null_test(c, nutot, e, rng, o) ? STEP_REAL : STEP_NULL;
+    return cold_null_test_env(c, nutot, envHi, envLo, e, rng, o) ? STEP_REAL : STEP_NULL;
 }
 
 template <typename V, bool THERMAL>

@@ -212,7 +212,7 @@ public:
     hipStream_t stream = nullptr;
     int I = 10000, P = 0, nCond = 0;
     int64_t capacity = 0, totalMain = 0;
-    int nChunks = 0, cus = 256;
+    int nChunks = 0, cus = 256, envShift = 0;
     std::vector<CondParams> hcp;
     std::vector<Chunk> hchunks;
     // device buffers
@@ -337,11 +337,13 @@ public:
             if (prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;
             if (prop.sharedMemPerBlock > 0) ldsMax = prop.sharedMemPerBlock;
         }
-        if ((size_t)(kBlock / kWave) * 3 * P * 8 > ldsMax)
+        if ((size_t)(kBlock / kWave) * (3 * P * 8 + 2 * kEnvBins * sizeof(V)) > ldsMax)
             return fail(EKMC_ERR_INVALID_ARGUMENT, "too many processes for the per-wave collision tallies in LDS");
         if (num.work_chunk < 0 || num.reserved != 0) return fail(EKMC_ERR_INVALID_ARGUMENT, "invalid work_chunk/reserved");
         int64_t chunkTarget = num.work_chunk > 0 ? num.work_chunk : kChunkDefault;
         chunkTarget = std::max<int64_t>(kWave, (chunkTarget + kWave - 1) / kWave * kWave);
+        envShift = 0;  // envelope bins of 2^envShift table rows, at most kEnvBins of them
+        while (((I - 1) >> envShift) >= kEnvBins) ++envShift;
         hcp.assign(n, CondParams());
         hchunks.clear();
         int64_t base = 0;
@@ -369,7 +371,8 @@ public:
             cp.N = cs[c].N;
             cp.gasEnergy = cs[c].gasEnergy;
             cp.seed = num.seed ^ ((uint64_t)c * 0x9E3779B97F4A7C15ull);
-            cp.tableOffset = (int64_t)c * I * (2 * P + 1);
+            cp.tableOffset = (int64_t)c * table_block_size(I, P);
+            cp.envShift = envShift;
             cp.base = base;
             cp.nMain = Nc;
             cp.regionBase = base + Nc;
@@ -401,7 +404,7 @@ public:
         HIP_TRY(alive.alloc(capacity));
         HIP_TRY(holeList.alloc(capacity));
         HIP_TRY(hipMemsetAsync(alive.p, 0, capacity, stream));
-        HIP_TRY(tables.alloc((size_t)n * I * (2 * P + 1)));
+        HIP_TRY(tables.alloc((size_t)n * table_block_size(I, P)));
         HIP_TRY(dconds.alloc(n));
         HIP_TRY(chunks.alloc(nChunks));
         HIP_TRY(hipMemcpyAsync(chunks.p, hchunks.data(), nChunks * sizeof(Chunk), hipMemcpyHostToDevice, stream));
@@ -561,7 +564,8 @@ public:
         tb.P = P;
         tb.maxEnergy = maxEnergy;
         tb.step = maxEnergy / (double)(I - 1);
-        HIP_TRY(launch_tables<V>(tsrc, cs[c].N, tb.step, tables.p + hcp[c].tableOffset, nutot64.p, numax64.p, stream));
+        HIP_TRY(launch_tables<V>(tsrc, cs[c].N, tb.step, envShift, tables.p + hcp[c].tableOffset, nutot64.p,
+                                 numax64.p, stream));
         HIP_TRY(hipMemcpyAsync(pinNumax.p, numax64.p, (size_t)I * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         tb.numax.assign(pinNumax.p, pinNumax.p + I);

@@ -125,10 +125,14 @@ struct DevTableSrc {
 
 // Build condition tables on the device for gas density N and energy step
 // `step` (= max energy / (I-1)): `tables` receives cumul [I*P], interp [I*P],
-// nutot [I] in V; nutot64 [I] and numax64 [I] (running maximum) in FP64.
+// nutot [I], then the null-test envelope envHi [kEnvBins], envLo [kEnvBins]
+// over bins of 1 << envShift rows, in V; nutot64 [I] and numax64 [I]
+// (running maximum) in FP64.
 template <typename V>
-hipError_t launch_tables(const DevTableSrc& src, double N, double step, V* tables, double* nutot64,
-                         double* numax64, hipStream_t st);
+hipError_t launch_tables(const DevTableSrc& src, double N, double step, int envShift, V* tables,
+                         double* nutot64, double* numax64, hipStream_t st);
+// Elements of one condition's table block.
+inline int64_t table_block_size(int I, int P) { return (int64_t)I * (2 * P + 1) + 2 * kEnvBins; }
 
 // mainBase[c]: index of condition c's first electron among all main electrons
 template <typename V>

@@ -53,6 +53,12 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 // collision operator for all of them.
 constexpr int kParkMin = 32;
 
+// LDS of one wave of the flight kernel: collision tallies [3P] (64-bit) and
+// the null-test envelope of the chunk's condition [2 kEnvBins] (V).
+template <typename V> __host__ __device__ constexpr size_t flight_wave_lds(int P) {
+    return (size_t)3 * P * 8 + (size_t)2 * kEnvBins * sizeof(V);
+}
+
 // ------------------------------------------------------------ wave helpers
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 // number of set bits of `mask` below this lane
@@ -285,7 +291,10 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
     const V* cumul = tables + cp.tableOffset;
     const V* interp = cumul + I * P;
     const V* nutot = interp + I * P;
+    V* envHi = (V*)(cnt + 3 * P);
+    V* envLo = envHi + kEnvBins;
     for (int i = lane; i < 3 * P; i += kWave) ((long long*)cnt)[i] = 0;
+    for (int i = lane; i < 2 * kEnvBins; i += kWave) envHi[i] = nutot[I + i];
     wave_lds_sync();
 
     const int count = ch.count;
@@ -341,7 +350,7 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
             } else {
                 double gain = 0;
                 EventRng<V> rng(cp.seed, e.stream, e.event);
-                const int st = step_a<V, FC, THERMAL>(cp, pr, nutot, e, rng, o, gain);
+                const int st = step_a<V, FC, THERMAL>(cp, pr, nutot, envHi, envLo, e, rng, o, gain);
                 fieldGain += gain;
                 if (st == STEP_DONE) {
                     done = true;
@@ -464,7 +473,7 @@ __global__ __launch_bounds__(kBlock) void k_advance(DevState<V> s, DevWork w, De
                                                     const int32_t* list, int32_t nList, uint32_t* queue) {
     extern __shared__ unsigned char smem[];
     const int wid = threadIdx.x / kWave;
-    unsigned long long* cnt = (unsigned long long*)smem + (size_t)wid * 3 * pr.P;
+    unsigned long long* cnt = (unsigned long long*)(smem + (size_t)wid * flight_wave_lds<V>(pr.P));
     while (true) {
         uint32_t item = 0;
         if (lane_id() == 0) item = atomicAdd(queue, 1u);
@@ -1024,10 +1033,34 @@ __global__ __launch_bounds__(1024) void k_table_max(const double* nutot64, doubl
     }
 }
 
+// Null-test envelope: for bin b (rows b<<s .. (b+1)<<s, the last row being
+// the partner of the bin's last interpolation), the largest and smallest
+// total frequency, widened by a few ulp of V so that every interpolated value
+// w1*a + w2*b of the bin's rows lies inside.
+template <typename V> struct EnvMargin;
+template <> struct EnvMargin<float> { static constexpr double up = 1.0 + 0x1p-20, down = 1.0 - 0x1p-20; };
+template <> struct EnvMargin<double> { static constexpr double up = 1.0 + 0x1p-48, down = 1.0 - 0x1p-48; };
+
 template <typename V>
-hipError_t launch_tables(const DevTableSrc& src, double N, double step, V* tables, double* nutot64,
+__global__ __launch_bounds__(kEnvBins) void k_table_env(V* tables, int I, int P, int shift) {
+    const V* nutot = tables + 2 * (size_t)I * P;
+    V* envHi = tables + 2 * (size_t)I * P + I;
+    V* envLo = envHi + kEnvBins;
+    const int b = threadIdx.x;
+    const int k0 = b << shift;
+    if (k0 > I - 1) { envHi[b] = (V)0; envLo[b] = (V)0; return; }
+    const int k1 = min(((b + 1) << shift), I - 1);
+    V hi = nutot[k0], lo = nutot[k0];
+    for (int k = k0 + 1; k <= k1; ++k) { hi = fmax(hi, nutot[k]); lo = fmin(lo, nutot[k]); }
+    envHi[b] = (V)((double)hi * EnvMargin<V>::up);
+    envLo[b] = (V)((double)lo * EnvMargin<V>::down);
+}
+
+template <typename V>
+hipError_t launch_tables(const DevTableSrc& src, double N, double step, int envShift, V* tables, double* nutot64,
                          double* numax64, hipStream_t st) {
     hipLaunchKernelGGL(k_table_rows<V>, dim3(grid_of(src.I)), dim3(kBlock), 0, st, src, N, step, tables, nutot64);
+    hipLaunchKernelGGL(k_table_env<V>, dim3(1), dim3(kEnvBins), 0, st, tables, src.I, src.P, envShift);
     hipLaunchKernelGGL(k_table_max, dim3(1), dim3(1024), 0, st, nutot64, numax64, src.I);
     return hipGetLastError();
 }
@@ -1091,7 +1124,7 @@ __global__ __launch_bounds__(kBlock) void k_op_philox(int64_t n, const uint32_t*
 
 // ------------------------------------------------------------- launchers
 static inline unsigned chunk_grid(int nChunks) { return (unsigned)((nChunks + kWavesPerBlock - 1) / kWavesPerBlock); }
-static inline size_t flight_lds(int P) { return (size_t)kWavesPerBlock * 3 * P * 8; }
+template <typename V> static inline size_t flight_lds(int P) { return (size_t)kWavesPerBlock * flight_wave_lds<V>(P); }
 
 template <typename V>
 hipError_t launch_init(const DevState<V>& s, const DevWork& w, int64_t totalMain, const int64_t* mainBase,
@@ -1112,7 +1145,7 @@ static void launch_advance_case(const DevState<V>& s, const DevWork& w, const De
                                 const FlightLists& fl, hipStream_t st) {
     if (fl.count[FC] == 0) return;
     dim3 g(fl.grid[FC]), b(kBlock);
-    const size_t lds = flight_lds(pr.P);
+    const size_t lds = flight_lds<V>(pr.P);
     uint32_t* q = w.queue + FC;
     if (pr.gasTempEffect == 0)
         hipLaunchKernelGGL((k_advance<V, FC, false>), g, b, lds, st, s, w, pr, tables, fl.list[FC], fl.count[FC], q);
@@ -1131,7 +1164,7 @@ int flight_blocks_per_cu(int fc, bool thermal, int P) {
         default: k = thermal ? flight_kernel<V, FIELD_AC_B, true>() : flight_kernel<V, FIELD_AC_B, false>(); break;
     }
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kBlock, flight_lds(P)) != hipSuccess || n < 1) n = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kBlock, flight_lds<V>(P)) != hipSuccess || n < 1) n = 1;
     return n;
 }
 
@@ -1227,7 +1260,8 @@ hipError_t launch_op_philox(int64_t n, const uint32_t* ctr, const uint32_t* key,
     template hipError_t launch_sample<V>(const DevState<V>&, const DevWork&, hipStream_t);                    \
     template hipError_t launch_hist<V>(const DevState<V>&, const DevHist&, hipStream_t);                     \
     template int flight_blocks_per_cu<V>(int, bool, int);                                                     \
-    template hipError_t launch_tables<V>(const DevTableSrc&, double, double, V*, double*, double*, hipStream_t); \
+    template hipError_t launch_tables<V>(const DevTableSrc&, double, double, int, V*, double*, double*,        \
+                                         hipStream_t);                                                       \
     template hipError_t launch_op_accelerate<V>(const CondParams*, int64_t, const double*, const double*,     \
                                                 double*, double*, double*, double*, hipStream_t);             \
     template hipError_t launch_op_collide<V>(const CondParams*, const DevProcesses&, const V*, int64_t,       \
