@@ -300,14 +300,14 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
     const int count = ch.count;
     int nextMain = 0, nq = 0, nextNew = 0, holes = 0;
     Electron<V> e;
-    bool have = false, isNew = false, holeOnLoad = false, parked = false;
+    bool have = false, isNew = false, parked = false;
     int64_t slot = 0;
-    uint8_t al = 1;
     uint32_t c0[4] = {0, 0, 0, 0};  // call 0 of a parked electron's event
     uint32_t iter = 0, nNull = 0, nReal = 0, nEsc = 0, status = 0;
     double fieldGain = 0, maxE = 0;
     while (true) {
         // lanes without an electron take the next main electron, then newborns
+        // (every main slot holds an alive electron after population control)
         const uint64_t need = __ballot(!have);
         if (need) {
             const int nNeed = __popcll(need);
@@ -328,8 +328,6 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
                 }
                 if (have) {
                     load_rec(s.rec, slot, e);
-                    al = isNew ? (uint8_t)1 : s.alive[slot];
-                    holeOnLoad = al == 0;
                     iter = 0;
                 }
             }
@@ -344,10 +342,7 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
         o.pid = -1;
         // phase A: flight, and the null test of the cold branch
         if (have && !parked) {
-            if (holeOnLoad) {
-                done = true;
-                dead = true;
-            } else {
+            {
                 double gain = 0;
                 EventRng<V> rng(cp.seed, e.stream, e.event);
                 const int st = step_a<V, FC, THERMAL>(cp, pr, nutot, envHi, envLo, e, rng, o, gain);
@@ -424,7 +419,7 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
                 if (!isNew) w.holeList[ch.first + holes + lane_rank(dm)] = (uint32_t)slot;
             } else {
                 store_rec(s.rec, slot, e);
-                if (al != 1) s.alive[slot] = 1;
+                if (!isNew) s.alive[slot] = 1;  // clears the clone mark of the last interval
                 maxE = fmax(maxE, (double)e.eps);
             }
             have = false;
