@@ -673,14 +673,15 @@ __device__ __forceinline__ int step_a(const CondParams& c, const DevProcesses& p
         e.tfree = free_time(rng.u(0), c);
         e.nue = (V)c.trial;
     }
-    if (e.t + (double)e.tfree > c.tsync) {
-        double dt = c.tsync - e.t;
-        flight<FC>(c, e, dt, gain);
+    // one flight for both outcomes (a wave holds both kinds of lanes)
+    const bool partial = e.t + (double)e.tfree > c.tsync;
+    const double dt = partial ? c.tsync - e.t : (double)e.tfree;
+    flight<FC>(c, e, dt, gain);
+    if (partial) {
         e.t = c.tsync;
         e.tfree = (V)((double)e.tfree - dt);
         return STEP_DONE;
     }
-    flight<FC>(c, e, (double)e.tfree, gain);
     e.t += (double)e.tfree;
     if (THERMAL && is_thermal(c, pr, e)) return STEP_REAL;
     return cold_null_test_env(c, nutot, envHi, envLo, e, rng, o) ? STEP_REAL : STEP_NULL;
