@@ -132,6 +132,10 @@ def main():
     ap.add_argument("--cpu-cap", type=int, default=20000)
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB")
+    ap.add_argument("--kernel-breakdown", action="store_true",
+                    help="time every kernel (HIP events around each launch cost ~4%% per step)")
+    ap.add_argument("--work-chunk", type=int, default=0,
+                    help="electrons per wavefront work unit (0: engine default)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
@@ -142,9 +146,12 @@ def main():
               "without communication, see DESIGN.md)." % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.kernel_breakdown:
+        os.environ["EKMC_KERNEL_TIMING"] = "all"
     mix, conds, num = ekmc.workload(WORKLOAD)
     num.precision = ekmc.MIXED
     num.seed = args.seed
+    num.work_chunk = args.work_chunk
     # every condition keeps running through warmup and the timed steps, so a
     # step is always one interval of the whole 64-condition batch
     num.n_integration_points = 1e15
@@ -157,8 +164,10 @@ def main():
     ms_per_step = 1e3 * secs / args.steps
     print("bench: batched mixed %.3e events/s (%d events in %.3f s)" % (value, events, secs),
           file=sys.stderr, flush=True)
-    adv_launches, adv_ms = stats["advance"]
-    adv_avg_ms = adv_ms / max(adv_launches, 1)
+    adv_launches, adv_ms = stats.get("advance", (0, 0.0))
+    if adv_launches == 0:
+        raise RuntimeError("no flight-kernel timings (EKMC_KERNEL_TIMING=0?): the roofline needs them")
+    adv_avg_ms = adv_ms / adv_launches
     # algorithmic bytes of one interval's flight launches: every electron's
     # record read once and written once (table reads hit L2/MALL, not counted)
     adv_bytes = float(n_total * interval_bytes(ekmc.MIXED))

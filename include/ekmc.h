@@ -281,7 +281,9 @@ typedef struct {
 } ekmc_cond_status;
 ekmc_status ekmc_get_status(const ekmc_engine* engine, int32_t cond, ekmc_cond_status* out);
 
-/* Per-kernel HIP-event timings since creation (or the last reset), followed
+/* Per-kernel HIP-event timings since creation (or the last reset): the flight
+ * kernels by default, every kernel when the environment variable
+ * EKMC_KERNEL_TIMING is "all", none when it is "0"; followed
  * by host-side phases (names starting with "host_"): preparing and launching
  * an interval, waiting for it, the per-condition control logic, table
  * rebuilds and histogram downloads. */

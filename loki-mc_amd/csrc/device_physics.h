@@ -546,54 +546,43 @@ __device__ __forceinline__ void apply_collision(const DevProcesses& pr, Electron
         o.dE = -(double)inc;  // attachmentCollision: the electron is lost
         return;
     }
-    if (type == 0) {
-        // conservativeCollision
+    if (THERMAL && thermal && type == 0) {
+        // conservativeCollision with the target's thermal velocity
         double M = pr.tMass[chosen], mu = pr.rMass[chosen], loss = pr.energyLoss[chosen];
-        if (THERMAL && thermal) {
-            Sph<V> s;
-            if (!cart2sph(e.vx - Vt[0], e.vy - Vt[1], e.vz - Vt[2], s)) { o.flags |= ST_ZERO_VELOCITY; o.pid = -1; return; }
-            double relE = 0.5 * mu * (double)s.norm * (double)s.norm / kCharge;
-            double relAfter = relE - loss;
-            if (relAfter <= 0) { o.pid = -1; return; }
-            V cc = cos_chi<V>(pr, chosen, (V)relE, (V)relAfter, rng, 6);
-            V sc = sqrt((V)1 - cc * cc);
-            V se, ce;
-            turn_sincos(rng.u(7), se, ce);
-            V u[3];
-            euler(sc, cc, se, ce, s, u);
-            double rel2 = (double)s.norm * (double)s.norm - 2.0 / mu * loss * kCharge;
-            if (!(rel2 > 0)) { o.pid = -1; return; }  // rounding at the threshold edge
-            V sp = (V)sqrt(rel2);
-            V fM = (V)(M / (kMass + M)), fe = (V)(kMass / (kMass + M));
-            e.vx = fM * (sp * u[0]) + (fe * e.vx + fM * Vt[0]);
-            e.vy = fM * (sp * u[1]) + (fe * e.vy + fM * Vt[1]);
-            e.vz = fM * (sp * u[2]) + (fe * e.vz + fM * Vt[2]);
-        } else {
-            Sph<V> s;
-            if (!cart2sph(e.vx, e.vy, e.vz, s)) { o.flags |= ST_ZERO_VELOCITY; o.pid = -1; return; }
-            double after = (double)inc - loss;
-            if (after <= 0) { o.pid = -1; return; }
-            V cc = cos_chi<V>(pr, chosen, inc, (V)after, rng, 6);
-            V sc = sqrt((V)1 - cc * cc);
-            V se, ce;
-            turn_sincos(rng.u(7), se, ce);
-            V u[3];
-            euler(sc, cc, se, ce, s, u);
-            double v2after = (double)s.norm * (double)s.norm - 2.0 / kMass * loss * kCharge;
-            if (!(v2after > 0)) { o.pid = -1; return; }  // rounding at the threshold edge
-            V sp = sqrt((V)v2after * ((V)1 - (V)(2.0 * mu / (kMass + M)) * ((V)1 - cc)));
-            e.vx = sp * u[0]; e.vy = sp * u[1]; e.vz = sp * u[2];
-        }
+        Sph<V> s;
+        if (!cart2sph(e.vx - Vt[0], e.vy - Vt[1], e.vz - Vt[2], s)) { o.flags |= ST_ZERO_VELOCITY; o.pid = -1; return; }
+        double relE = 0.5 * mu * (double)s.norm * (double)s.norm / kCharge;
+        double relAfter = relE - loss;
+        if (relAfter <= 0) { o.pid = -1; return; }
+        V cc = cos_chi<V>(pr, chosen, (V)relE, (V)relAfter, rng, 6);
+        V sc = sqrt((V)1 - cc * cc);
+        V se, ce;
+        turn_sincos(rng.u(7), se, ce);
+        V u[3];
+        euler(sc, cc, se, ce, s, u);
+        double rel2 = (double)s.norm * (double)s.norm - 2.0 / mu * loss * kCharge;
+        if (!(rel2 > 0)) { o.pid = -1; return; }  // rounding at the threshold edge
+        V sp = (V)sqrt(rel2);
+        V fM = (V)(M / (kMass + M)), fe = (V)(kMass / (kMass + M));
+        e.vx = fM * (sp * u[0]) + (fe * e.vx + fM * Vt[0]);
+        e.vy = fM * (sp * u[1]) + (fe * e.vy + fM * Vt[1]);
+        e.vz = fM * (sp * u[2]) + (fe * e.vz + fM * Vt[2]);
         e.eps = energy_of(e.vx, e.vy, e.vz);
         o.dE = (double)e.eps - (double)inc;
-    } else if (type == 1) {
-        // ionizationCollision
-        double thr = pr.energyLoss[chosen];
-        if ((double)inc < thr) { o.pid = -1; return; }
-        Sph<V> s;
-        if (!cart2sph(e.vx, e.vy, e.vz, s)) { o.flags |= ST_ZERO_VELOCITY; o.pid = -1; return; }
-        double netd = (double)inc - thr;
-        V net = (V)netd, ej;
+        return;
+    }
+    // conservativeCollision (cold) and ionizationCollision share the
+    // scattering of the incident electron: one sampler, azimuth and rotation
+    const bool ion = type == 1;
+    const double loss = pr.energyLoss[chosen];  // energy loss, or the ionization threshold
+    if (ion && (double)inc < loss) { o.pid = -1; return; }
+    Sph<V> s;
+    if (!cart2sph(e.vx, e.vy, e.vz, s)) { o.flags |= ST_ZERO_VELOCITY; o.pid = -1; return; }
+    const double after = (double)inc - loss;  // energy after the loss; net energy of an ionization
+    if (!ion && after <= 0) { o.pid = -1; return; }
+    const V net = (V)after;
+    V ej = (V)0, scE = net;
+    if (ion) {
         if (pr.ionizOp == 2) {
             V w = (V)pr.wParam[chosen];
             ej = w * tan(rng.u(8) * atan(net / ((V)2 * w)));
@@ -602,32 +591,48 @@ __device__ __forceinline__ void apply_collision(const DevProcesses& pr, Electron
         } else {
             ej = (V)pr.sharingFactor * net;
         }
-        V sc_e = net - ej;
-        V sCs, cCs, sCe, cCe, sEs, cEs, sEe, cEe;
-        if (pr.scat[chosen] == 5) {
-            cCs = sqrt(sc_e / net); sCs = sqrt((V)1 - cCs * cCs);
-            turn_sincos(rng.u(7), sEs, cEs);
-            cCe = sqrt(ej / net); sCe = sqrt((V)1 - cCe * cCe);
-            sEe = -sEs; cEe = -cEs;
-        } else {
-            cCs = cos_chi<V>(pr, chosen, inc, sc_e, rng, 6);
-            sCs = sqrt((V)1 - cCs * cCs);
-            turn_sincos(rng.u(7), sEs, cEs);
-            cCe = cos_chi<V>(pr, chosen, inc, ej, rng, 9);
-            sCe = sqrt((V)1 - cCe * cCe);
-            turn_sincos(rng.u(10), sEe, cEe);
-        }
-        V u[3];
-        euler(sCs, cCs, sEs, cEs, s, u);
-        V sp = sqrt((V)2 * sc_e * (V)kCharge / (V)kMass);
-        e.vx = sp * u[0]; e.vy = sp * u[1]; e.vz = sp * u[2];
-        euler(sCe, cCe, sEe, cEe, s, u);
-        V spE = sqrt((V)2 * ej * (V)kCharge / (V)kMass);
-        o.ejv[0] = spE * u[0]; o.ejv[1] = spE * u[1]; o.ejv[2] = spE * u[2];
-        o.ejEps = ej;
-        e.eps = sc_e;
-        o.dE = -thr;
+        scE = net - ej;
     }
+    const bool boeuf = ion && pr.scat[chosen] == 5;
+    V cc;
+    if (boeuf) cc = sqrt(scE / net);
+    else cc = cos_chi<V>(pr, chosen, inc, scE, rng, 6);
+    const V sc = sqrt((V)1 - cc * cc);
+    V se, ce;
+    turn_sincos(rng.u(7), se, ce);
+    V u[3];
+    euler(sc, cc, se, ce, s, u);
+    V sp;
+    if (ion) {
+        sp = sqrt((V)2 * scE * (V)kCharge / (V)kMass);
+    } else {
+        const double M = pr.tMass[chosen], mu = pr.rMass[chosen];
+        double v2after = (double)s.norm * (double)s.norm - 2.0 / kMass * loss * kCharge;
+        if (!(v2after > 0)) { o.pid = -1; return; }  // rounding at the threshold edge
+        sp = sqrt((V)v2after * ((V)1 - (V)(2.0 * mu / (kMass + M)) * ((V)1 - cc)));
+    }
+    e.vx = sp * u[0]; e.vy = sp * u[1]; e.vz = sp * u[2];
+    if (!ion) {
+        e.eps = energy_of(e.vx, e.vy, e.vz);
+        o.dE = (double)e.eps - (double)inc;
+        return;
+    }
+    // the ejected electron (ionizationCollision, BoltzmannMC.C:1228-1267)
+    V cCe, sCe, sEe, cEe;
+    if (boeuf) {
+        cCe = sqrt(ej / net); sCe = sqrt((V)1 - cCe * cCe);
+        sEe = -se; cEe = -ce;
+    } else {
+        cCe = cos_chi<V>(pr, chosen, inc, ej, rng, 9);
+        sCe = sqrt((V)1 - cCe * cCe);
+        turn_sincos(rng.u(10), sEe, cEe);
+    }
+    euler(sCe, cCe, sEe, cEe, s, u);
+    V spE = sqrt((V)2 * ej * (V)kCharge / (V)kMass);
+    o.ejv[0] = spE * u[0]; o.ejv[1] = spE * u[1]; o.ejv[2] = spE * u[2];
+    o.ejEps = ej;
+    e.eps = scE;
+    o.dE = -loss;
 }
 
 template <typename V> __device__ __forceinline__ void clear_out(CollOut<V>& o) {

@@ -16,6 +16,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -217,7 +218,7 @@ public:
     std::vector<Chunk> hchunks;
     // device buffers
     DevBuf<Rec<V>> rec;
-    DevBuf<uint32_t> holeList, status, queue;
+    DevBuf<uint32_t> holeList, markKeys, status, queue;
     DevBuf<uint8_t> alive;
     DevBuf<V> tables;
     DevBuf<CondParams> dconds;
@@ -256,7 +257,7 @@ public:
             if (timer.start[k]) (void)hipEventDestroy(timer.start[k]);
             if (timer.stop[k]) (void)hipEventDestroy(timer.stop[k]);
         }
-        rec.release(); holeList.release(); status.release(); queue.release();
+        rec.release(); holeList.release(); markKeys.release(); status.release(); queue.release();
         alive.release(); tables.release(); dconds.release(); chunks.release(); holesC.release();
         surplusC.release(); markC.release(); flist.release(); phaseRow.release(); cylFlag.release();
         holeOff.release(); surOff.release(); markOff.release(); mainBase.release(); tailC.release();
@@ -292,7 +293,10 @@ public:
             HIP_TRY(hipEventCreate(&timer.stop[k]));
             timer.armed[k] = false;
         }
-        timer.enabled = true;
+        // HIP-event timing of the flight kernels; EKMC_KERNEL_TIMING=all times
+        // every kernel, EKMC_KERNEL_TIMING=0 none
+        const char* kt = std::getenv("EKMC_KERNEL_TIMING");
+        timer.level = !kt ? 1 : (std::strcmp(kt, "all") == 0 ? 2 : (kt[0] == '0' ? 0 : 1));
         kstat.assign(K_COUNT + H_COUNT, {0, 0.0});
         cs.assign(n, Cond());
         int64_t sumN = 0;
@@ -403,6 +407,7 @@ public:
         HIP_TRY(rec.alloc(capacity));
         HIP_TRY(alive.alloc(capacity));
         HIP_TRY(holeList.alloc(capacity));
+        HIP_TRY(markKeys.alloc(capacity));
         HIP_TRY(hipMemsetAsync(alive.p, 0, capacity, stream));
         HIP_TRY(tables.alloc((size_t)n * table_block_size(I, P)));
         HIP_TRY(dconds.alloc(n));
@@ -510,7 +515,7 @@ public:
         ds = DevState<V>{rec.p, alive.p};
         dw.conds = dconds.p; dw.chunks = chunks.p; dw.holesC = holesC.p; dw.surplusC = surplusC.p;
         dw.markC = markC.p; dw.holeOff = holeOff.p; dw.surOff = surOff.p; dw.markOff = markOff.p;
-        dw.holeList = holeList.p; dw.tailC = tailC.p; dw.momC = momC.p; dw.accCount = accCount.p;
+        dw.holeList = holeList.p; dw.markKeys = markKeys.p; dw.tailC = tailC.p; dw.momC = momC.p; dw.accCount = accCount.p;
         dw.accGain = accGain.p; dw.accLoss = accLoss.p; dw.plans = plans.p;
         dw.status = status.p; dw.queue = queue.p; dw.summary = summary.p; dw.summarySize = summarySize;
         dw.recSize = recSize; dw.nChunks = nChunks; dw.nCond = n;

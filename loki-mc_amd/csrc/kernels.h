@@ -60,6 +60,7 @@ struct DevWork {
     int64_t* surOff;
     int64_t* markOff;
     uint32_t* holeList;           // [pool capacity]; a chunk's holes at holeList[first ..)
+    uint32_t* markKeys;           // [pool capacity]; removal keys of a condition's marks at [base ..)
     double* tailC;                // [n_chunks * CT_SIZE]
     double* momC;                 // [n_chunks * kMomentRecord]
     unsigned long long* accCount; // [n_cond * P] real collisions per process
@@ -168,7 +169,7 @@ extern const char* kKernelNames[K_COUNT];
 struct KernelTimer {
     hipEvent_t start[K_COUNT], stop[K_COUNT];
     bool armed[K_COUNT];
-    bool enabled;
+    int level;  // 0 none, 1 flight kernels, 2 every kernel
 };
 void set_kernel_timer(KernelTimer* t);
 

@@ -39,11 +39,13 @@ const char* kKernelNames[K_COUNT] = {"init", "advance", "plan", "marks", "apply_
 static KernelTimer* g_timer = nullptr;
 void set_kernel_timer(KernelTimer* t) { g_timer = t; }
 
+// level 1 times the flight kernels only, level 2 every kernel
+static inline bool timed(int k) { return g_timer && (g_timer->level >= 2 || (g_timer->level == 1 && k == K_ADVANCE)); }
 static inline void timer_start(int k, hipStream_t st) {
-    if (g_timer && g_timer->enabled) { (void)hipEventRecord(g_timer->start[k], st); }
+    if (timed(k)) { (void)hipEventRecord(g_timer->start[k], st); }
 }
 static inline void timer_stop(int k, hipStream_t st) {
-    if (g_timer && g_timer->enabled) { (void)hipEventRecord(g_timer->stop[k], st); g_timer->armed[k] = true; }
+    if (timed(k)) { (void)hipEventRecord(g_timer->stop[k], st); g_timer->armed[k] = true; }
 }
 
 constexpr double kFix = 16777216.0;  // 2^24: fixed-point scale of energy sums [eV]
@@ -590,13 +592,16 @@ __global__ __launch_bounds__(1024) void k_plan_marks(DevWork w) {
     }
     if (threadIdx.x == 0) { sh[0] = 0; sh[1] = 0; sh[2] = (uint32_t)R; sh[3] = 0; }
     __syncthreads();
+    uint32_t* keys = w.markKeys + cp.base;  // K <= the condition's pool slots
     for (int pass = 0; pass < 4; ++pass) {
         const int shift = 24 - 8 * pass;
         for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
         __syncthreads();
         const uint32_t prefix = sh[0], mask = sh[1];
         for (int64_t j = threadIdx.x; j < K; j += blockDim.x) {
-            const uint32_t k = mark_key(cp, j);
+            uint32_t k;
+            if (pass == 0) { k = mark_key(cp, j); keys[j] = k; }
+            else k = keys[j];
             if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
         }
         __syncthreads();
@@ -615,7 +620,7 @@ __global__ __launch_bounds__(1024) void k_plan_marks(DevWork w) {
     }
     const uint32_t T = sh[0], need = sh[2];  // R-th smallest key; its rank among keys == T
     for (int64_t j = threadIdx.x; j < K; j += blockDim.x) {
-        if (mark_key(cp, j) == T) {
+        if (keys[j] == T) {
             const uint32_t slot = atomicAdd(&sh[3], 1u);
             if (slot < kTieCap) ties[slot] = (uint32_t)j;
         }
@@ -688,7 +693,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_marks(DevState<V> s, DevWork w
         int64_t slot = 0;
         if (more) {
             if (mark_alive(s.alive, ch, g.pos)) {
-                const uint32_t k = mark_key(cp, gi);
+                const uint32_t k = w.markKeys[cp.base + gi];
                 if (k < pl.thrKey || (k == pl.thrKey && gi <= pl.thrIndex)) {
                     kill = true;
                     killMain = g.pos < ch.count;
