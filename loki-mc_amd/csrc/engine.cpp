@@ -231,6 +231,8 @@ public:
     DevBuf<PopPlan> plans;
     FlightLists fl = {};
     std::vector<int32_t> listActive;  // active flags the flight lists were built for
+    bool histFlagsOnDevice = false;   // the device grid holds fill flags of the last interval
+    bool fusedHist = true;            // running conditions' histograms counted by the flight kernel
     DevBuf<int> iproc;
     DevBuf<double> dproc;
     DevBuf<unsigned long long> hEedf, hPer, hEadf, hEvdf;
@@ -293,6 +295,9 @@ public:
             HIP_TRY(hipEventCreate(&timer.stop[k]));
             timer.armed[k] = false;
         }
+        // EKMC_FUSED_HIST=0: every histogram by the histogram kernel
+        const char* fh = std::getenv("EKMC_FUSED_HIST");
+        fusedHist = !(fh && fh[0] == '0');
         // HIP-event timing of the flight kernels; EKMC_KERNEL_TIMING=all times
         // every kernel, EKMC_KERNEL_TIMING=0 none
         const char* kt = std::getenv("EKMC_KERNEL_TIMING");
@@ -341,7 +346,7 @@ public:
             if (prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;
             if (prop.sharedMemPerBlock > 0) ldsMax = prop.sharedMemPerBlock;
         }
-        if ((size_t)(kBlock / kWave) * (3 * P * 8 + 2 * kEnvBins * sizeof(V)) > ldsMax)
+        if ((size_t)(kBlock / kWave) * (3 * P * 8 + 2 * kEnvBins * sizeof(V) + 4 * std::min(num.n_energy_cells, 2048)) > ldsMax)
             return fail(EKMC_ERR_INVALID_ARGUMENT, "too many processes for the per-wave collision tallies in LDS");
         if (num.work_chunk < 0 || num.reserved != 0) return fail(EKMC_ERR_INVALID_ARGUMENT, "invalid work_chunk/reserved");
         int64_t chunkTarget = num.work_chunk > 0 ? num.work_chunk : kChunkDefault;
@@ -522,6 +527,7 @@ public:
         dh.eedf = hEedf.p; dh.eedfPer = hPer.p; dh.eadf = hEadf.p; dh.evdf = hEvdf.p; dh.grid = grid.p;
         dh.phaseRow = phaseRow.p; dh.cyl = cylFlag.p;
         dh.nCells = nCells; dh.nCos = nCos; dh.nRad = nRad; dh.nAx = nAx; dh.nPhases = nPhases;
+        dh.histLds = nCells <= 2048 ? nCells : 0;
         // first main electron of each condition, for the init kernel
         std::vector<int64_t> mb(n);
         std::vector<double> td(n);
@@ -638,7 +644,7 @@ public:
             fl.list[fc] = flist.p + begin;
             fl.count[fc] = count;
             if (count) {
-                const int64_t waves = (int64_t)flight_blocks_per_cu<V>(fc, thermal, P) * cus;
+                const int64_t waves = (int64_t)flight_blocks_per_cu<V>(fc, thermal, P, dh.histLds) * cus;
                 const int64_t need = (count + kBlock / kWave - 1) / (kBlock / kWave);
                 fl.grid[fc] = (int32_t)std::max<int64_t>(1, std::min<int64_t>(need, waves));
             }
@@ -904,7 +910,7 @@ public:
         for (int s = 0; s < nsync; ++s) {
             const auto tPrep = Clock::now();
             int active = 0;
-            bool anyHist = false;
+            bool anyHist = false, anyHistKernel = false;
             std::vector<double> hgrid((size_t)nCond * 6, 0.0);
             std::vector<int32_t> hrow(nCond, -1);
             for (int c = 0; c < nCond; ++c) {
@@ -912,7 +918,15 @@ public:
                 hcp[c].active = C.finished ? 0 : 1;
                 double* gr = &hgrid[(size_t)c * 6];
                 gr[0] = C.eedfStep; gr[1] = C.cosStep; gr[2] = C.radStep; gr[3] = C.axStep; gr[4] = C.maxSpeed;
-                if (C.histRequest) { gr[5] = 1; anyHist = true; hrow[c] = C.histPhase; C.histRequest = 0; }
+                if (C.histRequest) {
+                    // a running condition's flight kernel counts the state as it
+                    // loads it (flag 2); a finished one needs the histogram kernel
+                    gr[5] = (C.finished || !fusedHist) ? 1 : 2;
+                    anyHist = true;
+                    anyHistKernel |= C.finished || !fusedHist;
+                    hrow[c] = C.histPhase;
+                    C.histRequest = 0;
+                }
                 if (C.finished) continue;
                 ++active;
                 {
@@ -930,11 +944,12 @@ public:
                 hcp[c].syncIndex = C.syncIndex;
             }
             if (active == 0 && !anyHist) break;
-            if (anyHist) {
+            if (anyHist || histFlagsOnDevice) {
                 // histograms of the state sampled at the end of the previous interval
                 HIP_TRY(hipMemcpyAsync(grid.p, hgrid.data(), hgrid.size() * sizeof(double), hipMemcpyHostToDevice, stream));
                 HIP_TRY(hipMemcpyAsync(phaseRow.p, hrow.data(), nCond * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-                HIP_TRY(launch_hist<V>(ds, dh, stream));
+                histFlagsOnDevice = anyHist;
+                if (anyHistKernel) HIP_TRY(launch_hist<V>(ds, dh, stream));
             }
             if (active == 0) { HIP_TRY(hipStreamSynchronize(stream)); break; }
             {
@@ -943,7 +958,7 @@ public:
             }
             std::memcpy(pinCond.p, hcp.data(), nCond * sizeof(CondParams));
             HIP_TRY(hipMemcpyAsync(dconds.p, pinCond.p, nCond * sizeof(CondParams), hipMemcpyHostToDevice, stream));
-            HIP_TRY(launch_interval<V>(ds, dw, dpr, tables.p, fl, stream));
+            HIP_TRY(launch_interval<V>(ds, dw, dpr, tables.p, dh, fl, stream));
             HIP_TRY(launch_sample<V>(ds, dw, stream));
             host_add(H_PREPARE, ms_since(tPrep));
             ekmc_status st = run_reduce_and_fetch();
@@ -1045,10 +1060,11 @@ public:
             gr[0] = C.eedfStep; gr[1] = C.cosStep; gr[2] = C.radStep; gr[3] = C.axStep; gr[4] = C.maxSpeed;
             if (C.histRequest) { gr[5] = 1; any = true; hrow[c] = C.histPhase; C.histRequest = 0; }
         }
-        if (any) {
+        if (any || histFlagsOnDevice) {
             HIP_TRY(hipMemcpyAsync(grid.p, hgrid.data(), hgrid.size() * sizeof(double), hipMemcpyHostToDevice, stream));
             HIP_TRY(hipMemcpyAsync(phaseRow.p, hrow.data(), nCond * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-            HIP_TRY(launch_hist<V>(ds, dh, stream));
+            histFlagsOnDevice = any;
+            if (any) HIP_TRY(launch_hist<V>(ds, dh, stream));
         }
         HIP_TRY(hipStreamSynchronize(stream));
         sync_timers();

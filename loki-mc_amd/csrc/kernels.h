@@ -85,7 +85,9 @@ struct SlotRange {
 
 // Histogram accumulators (integer counts summed over samples) and the grid of
 // each condition.  grid[6c..6c+6) = {eedf step, cos step, radial step, axial
-// step, max speed, fill flag}; phaseRow[c] is the AC phase row or -1.
+// step, max speed, fill flag}; the fill flag is 1 for the histogram kernel, 2
+// for the flight kernel (counting main electrons as it loads them), 0 for no
+// histogram; phaseRow[c] is the AC phase row or -1.
 struct DevHist {
     unsigned long long* eedf;     // [n_cond * nCells]
     unsigned long long* eedfPer;  // [n_cond * nPhases * nCells] or null
@@ -97,6 +99,7 @@ struct DevHist {
     const SlotRange* ranges;      // [nRanges] main-slot ranges, one block each
     int nRanges;
     int nCells, nCos, nRad, nAx, nPhases;
+    int histLds;                  // EEDF cells the flight kernel counts in LDS (0: in HBM)
 };
 
 // Flight-kernel work lists, one per field case: chunk ids in processing order.
@@ -141,7 +144,7 @@ hipError_t launch_init(const DevState<V>& s, const DevWork& w, int64_t totalMain
                        const int64_t* mainBase, const double* thermalDev, hipStream_t st);
 template <typename V>
 hipError_t launch_interval(const DevState<V>& s, const DevWork& w, const DevProcesses& pr,
-                           const V* tables, const FlightLists& fl, hipStream_t st);
+                           const V* tables, const DevHist& h, const FlightLists& fl, hipStream_t st);
 template <typename V>
 hipError_t launch_sample(const DevState<V>& s, const DevWork& w, hipStream_t st);
 template <typename V>
@@ -149,7 +152,7 @@ hipError_t launch_hist(const DevState<V>& s, const DevHist& h, hipStream_t st);
 hipError_t launch_finish(const DevWork& w, int P, hipStream_t st);
 // Resident blocks per CU of the flight kernel of field case fc.
 template <typename V>
-int flight_blocks_per_cu(int fc, bool thermal, int P);
+int flight_blocks_per_cu(int fc, bool thermal, int P, int histLds);
 
 template <typename V>
 hipError_t launch_op_accelerate(const CondParams* cond, int64_t n, const double* time, const double* dt,

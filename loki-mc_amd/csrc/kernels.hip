@@ -55,10 +55,11 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 // collision operator for all of them.
 constexpr int kParkMin = 32;
 
-// LDS of one wave of the flight kernel: collision tallies [3P] (64-bit) and
-// the null-test envelope of the chunk's condition [2 kEnvBins] (V).
-template <typename V> __host__ __device__ constexpr size_t flight_wave_lds(int P) {
-    return (size_t)3 * P * 8 + (size_t)2 * kEnvBins * sizeof(V);
+// LDS of one wave of the flight kernel: collision tallies [3P] (64-bit), the
+// null-test envelope of the chunk's condition [2 kEnvBins] (V) and the EEDF
+// counts of a histogram taken at load [histLds] (32-bit).
+template <typename V> __host__ __device__ constexpr size_t flight_wave_lds(int P, int histLds) {
+    return (size_t)3 * P * 8 + (size_t)2 * kEnvBins * sizeof(V) + (size_t)histLds * 4;
 }
 
 // ------------------------------------------------------------ wave helpers
@@ -276,12 +277,43 @@ __device__ void wave_refill(const DevState<V>& s, uint32_t* holeList, const Chun
     remaining = used - filled;
 }
 
+// ------------------------------------------------------------- histograms
+// One electron into the histograms of condition c (getTimeDependDistributions,
+// BoltzmannMC.C:1551-1571; MathFunctions::histogramCount/histogram2DCount,
+// MathFunctions.C:61-81,106-127): the EEDF into `lhist` (LDS, flushed later)
+// or directly into HBM, the cylindrical EADF and EVDF into HBM.
+template <typename V>
+__device__ __forceinline__ void hist_count(const DevHist& h, int c, const double* g, const Electron<V>& e,
+                                           uint32_t* lhist) {
+    const double en = (double)e.eps;
+    const int cell = (int)(en / g[0]);
+    if (cell >= 0 && cell < h.nCells) {
+        if (lhist) atomicAdd(&lhist[cell], 1u);
+        else {
+            const int row = h.phaseRow[c];
+            atomicAdd(&h.eedf[(size_t)c * h.nCells + cell], 1ull);
+            if (row >= 0 && h.eedfPer) atomicAdd(&h.eedfPer[((size_t)c * h.nPhases + row) * h.nCells + cell], 1ull);
+        }
+    }
+    if (h.cyl[c] != 0 && h.eadf) {
+        const double vx = e.vx, vy = e.vy, vz = e.vz;
+        const double cosA = vz / sqrt(vx * vx + vy * vy + vz * vz);
+        const int cy = (int)((cosA + 1.0) / g[1]);
+        if (cell < h.nCells && cell >= 0 && cy < h.nCos && cy >= 0)
+            atomicAdd(&h.eadf[((size_t)c * h.nCells + cell) * h.nCos + cy], 1ull);
+        const int rx = (int)(sqrt(vx * vx + vy * vy) / g[2]);
+        const int az = (int)((vz + g[4]) / g[3]);
+        if (rx < h.nRad && rx >= 0 && az < h.nAx && az >= 0)
+            atomicAdd(&h.evdf[((size_t)c * h.nRad + rx) * h.nAx + az], 1ull);
+    }
+}
+
 // --------------------------------------------------------------- advance
 // One chunk: every main electron and every newborn flies to t_sync (the
 // passes of BoltzmannMC.C:630-681 for this set of electrons).
 template <typename V, int FC, bool THERMAL>
 __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWork& w, const DevProcesses& pr,
-                                              const V* tables, int cid, unsigned long long* cnt) {
+                                              const V* tables, const DevHist& h, int cid, unsigned long long* cnt) {
     const int lane = lane_id();
     const int P = pr.P;
     long long* gainL = (long long*)(cnt + P);
@@ -295,8 +327,16 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
     const V* nutot = interp + I * P;
     V* envHi = (V*)(cnt + 3 * P);
     V* envLo = envHi + kEnvBins;
+    uint32_t* lhist = (uint32_t*)(envLo + kEnvBins);
     for (int i = lane; i < 3 * P; i += kWave) ((long long*)cnt)[i] = 0;
     for (int i = lane; i < 2 * kEnvBins; i += kWave) envHi[i] = nutot[I + i];
+    // histograms of the sampled state (the state this interval starts from),
+    // counted as main electrons are loaded (fill flag 2 of the condition)
+    const double* hg = h.grid + 6 * (size_t)c;
+    const bool histOn = hg[5] == 2.0;
+    const bool histLds = histOn && h.histLds >= h.nCells;
+    if (histLds)
+        for (int i = lane; i < h.nCells; i += kWave) lhist[i] = 0;
     wave_lds_sync();
 
     const int count = ch.count;
@@ -331,6 +371,7 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
                 if (have) {
                     load_rec(s.rec, slot, e);
                     iter = 0;
+                    if (histOn && !isNew) hist_count<V>(h, c, hg, e, histLds ? lhist : nullptr);
                 }
             }
             nextMain += takeMain;
@@ -460,6 +501,16 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
             if (lq) atomicAdd((unsigned long long*)&w.accLoss[(size_t)c * P + p], (unsigned long long)lq);
         }
     }
+    if (histLds) {
+        const int row = h.phaseRow[c];
+        for (int i = lane; i < h.nCells; i += kWave) {
+            const uint32_t n = lhist[i];
+            if (!n) continue;
+            atomicAdd(&h.eedf[(size_t)c * h.nCells + i], (unsigned long long)n);
+            if (row >= 0 && h.eedfPer)
+                atomicAdd(&h.eedfPer[((size_t)c * h.nPhases + row) * h.nCells + i], (unsigned long long)n);
+        }
+    }
     wave_lds_sync();
 }
 
@@ -467,16 +518,17 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
 // the list through an atomic counter until the list is exhausted.
 template <typename V, int FC, bool THERMAL>
 __global__ __launch_bounds__(kBlock) void k_advance(DevState<V> s, DevWork w, DevProcesses pr, const V* tables,
-                                                    const int32_t* list, int32_t nList, uint32_t* queue) {
+                                                    DevHist h, const int32_t* list, int32_t nList,
+                                                    uint32_t* queue) {
     extern __shared__ unsigned char smem[];
     const int wid = threadIdx.x / kWave;
-    unsigned long long* cnt = (unsigned long long*)(smem + (size_t)wid * flight_wave_lds<V>(pr.P));
+    unsigned long long* cnt = (unsigned long long*)(smem + (size_t)wid * flight_wave_lds<V>(pr.P, h.histLds));
     while (true) {
         uint32_t item = 0;
         if (lane_id() == 0) item = atomicAdd(queue, 1u);
         item = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl((int)item, 0, kWave));
         if (item >= (uint32_t)nList) break;
-        advance_chunk<V, FC, THERMAL>(s, w, pr, tables, list[item], cnt);
+        advance_chunk<V, FC, THERMAL>(s, w, pr, tables, h, list[item], cnt);
     }
 }
 
@@ -852,7 +904,7 @@ __global__ __launch_bounds__(kBlock) void k_hist(DevState<V> s, const SlotRange*
     const SlotRange rg = ranges[blockIdx.x];
     const int c = rg.cond;
     const double* g = h.grid + 6 * (size_t)c;
-    if (g[5] == 0) return;
+    if (g[5] != 1.0) return;  // flag 2: counted by the flight kernel as it loads the state
     const bool useLds = ldsCells >= h.nCells;
     uint32_t* lhist = (uint32_t*)smem;
     if (useLds) {
@@ -1124,7 +1176,9 @@ __global__ __launch_bounds__(kBlock) void k_op_philox(int64_t n, const uint32_t*
 
 // ------------------------------------------------------------- launchers
 static inline unsigned chunk_grid(int nChunks) { return (unsigned)((nChunks + kWavesPerBlock - 1) / kWavesPerBlock); }
-template <typename V> static inline size_t flight_lds(int P) { return (size_t)kWavesPerBlock * flight_wave_lds<V>(P); }
+template <typename V> static inline size_t flight_lds(int P, int histLds) {
+    return (size_t)kWavesPerBlock * flight_wave_lds<V>(P, histLds);
+}
 
 template <typename V>
 hipError_t launch_init(const DevState<V>& s, const DevWork& w, int64_t totalMain, const int64_t* mainBase,
@@ -1142,19 +1196,19 @@ static const void* flight_kernel() { return (const void*)k_advance<V, FC, TH>; }
 
 template <typename V, int FC>
 static void launch_advance_case(const DevState<V>& s, const DevWork& w, const DevProcesses& pr, const V* tables,
-                                const FlightLists& fl, hipStream_t st) {
+                                const DevHist& h, const FlightLists& fl, hipStream_t st) {
     if (fl.count[FC] == 0) return;
     dim3 g(fl.grid[FC]), b(kBlock);
-    const size_t lds = flight_lds<V>(pr.P);
+    const size_t lds = flight_lds<V>(pr.P, h.histLds);
     uint32_t* q = w.queue + FC;
     if (pr.gasTempEffect == 0)
-        hipLaunchKernelGGL((k_advance<V, FC, false>), g, b, lds, st, s, w, pr, tables, fl.list[FC], fl.count[FC], q);
+        hipLaunchKernelGGL((k_advance<V, FC, false>), g, b, lds, st, s, w, pr, tables, h, fl.list[FC], fl.count[FC], q);
     else
-        hipLaunchKernelGGL((k_advance<V, FC, true>), g, b, lds, st, s, w, pr, tables, fl.list[FC], fl.count[FC], q);
+        hipLaunchKernelGGL((k_advance<V, FC, true>), g, b, lds, st, s, w, pr, tables, h, fl.list[FC], fl.count[FC], q);
 }
 
 template <typename V>
-int flight_blocks_per_cu(int fc, bool thermal, int P) {
+int flight_blocks_per_cu(int fc, bool thermal, int P, int histLds) {
     const void* k = nullptr;
     switch (fc) {
         case FIELD_DC: k = thermal ? flight_kernel<V, FIELD_DC, true>() : flight_kernel<V, FIELD_DC, false>(); break;
@@ -1164,20 +1218,20 @@ int flight_blocks_per_cu(int fc, bool thermal, int P) {
         default: k = thermal ? flight_kernel<V, FIELD_AC_B, true>() : flight_kernel<V, FIELD_AC_B, false>(); break;
     }
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kBlock, flight_lds<V>(P)) != hipSuccess || n < 1) n = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kBlock, flight_lds<V>(P, histLds)) != hipSuccess || n < 1) n = 1;
     return n;
 }
 
 template <typename V>
 hipError_t launch_interval(const DevState<V>& s, const DevWork& w, const DevProcesses& pr, const V* tables,
-                           const FlightLists& fl, hipStream_t st) {
+                           const DevHist& h, const FlightLists& fl, hipStream_t st) {
     const dim3 g(chunk_grid(w.nChunks)), b(kBlock);
     timer_start(K_ADVANCE, st);
-    launch_advance_case<V, FIELD_DC>(s, w, pr, tables, fl, st);
-    launch_advance_case<V, FIELD_AC>(s, w, pr, tables, fl, st);
-    launch_advance_case<V, FIELD_DC_B>(s, w, pr, tables, fl, st);
-    launch_advance_case<V, FIELD_ECR>(s, w, pr, tables, fl, st);
-    launch_advance_case<V, FIELD_AC_B>(s, w, pr, tables, fl, st);
+    launch_advance_case<V, FIELD_DC>(s, w, pr, tables, h, fl, st);
+    launch_advance_case<V, FIELD_AC>(s, w, pr, tables, h, fl, st);
+    launch_advance_case<V, FIELD_DC_B>(s, w, pr, tables, h, fl, st);
+    launch_advance_case<V, FIELD_ECR>(s, w, pr, tables, h, fl, st);
+    launch_advance_case<V, FIELD_AC_B>(s, w, pr, tables, h, fl, st);
     timer_stop(K_ADVANCE, st);
     timer_start(K_PLAN, st);
     hipLaunchKernelGGL(k_plan, dim3(w.nCond), dim3(1024), 0, st, w);
@@ -1256,10 +1310,10 @@ hipError_t launch_op_philox(int64_t n, const uint32_t* ctr, const uint32_t* key,
     template hipError_t launch_init<V>(const DevState<V>&, const DevWork&, int64_t, const int64_t*,           \
                                        const double*, hipStream_t);                                           \
     template hipError_t launch_interval<V>(const DevState<V>&, const DevWork&, const DevProcesses&, const V*, \
-                                           const FlightLists&, hipStream_t);                                  \
+                                           const DevHist&, const FlightLists&, hipStream_t);                  \
     template hipError_t launch_sample<V>(const DevState<V>&, const DevWork&, hipStream_t);                    \
     template hipError_t launch_hist<V>(const DevState<V>&, const DevHist&, hipStream_t);                     \
-    template int flight_blocks_per_cu<V>(int, bool, int);                                                     \
+    template int flight_blocks_per_cu<V>(int, bool, int, int);                                                \
     template hipError_t launch_tables<V>(const DevTableSrc&, double, double, int, V*, double*, double*,        \
                                          hipStream_t);                                                       \
     template hipError_t launch_op_accelerate<V>(const CondParams*, int64_t, const double*, const double*,     \

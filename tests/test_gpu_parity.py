@@ -337,6 +337,46 @@ def test_large_removal_sets_keep_n(prec, n):
     assert np.array_equal(out[0], out[1])
 
 
+def _distributions(eng, c, num, cyl, ac):
+    out = [eng.eedf(c)[1].copy()]
+    L = ekmc.lib()
+    if cyl:
+        nr, na, nc = num.n_radial_velocity_cells, num.n_axial_velocity_cells, num.n_energy_cells
+        r, a, f = np.zeros(nr), np.zeros(na), np.zeros(nr * na)
+        f1, f2 = np.zeros(nc), np.zeros(nc)
+        ekmc.check(L.ekmc_get_evdf(eng.h, c, ekmc.dptr(r), ekmc.dptr(a), ekmc.dptr(f), ekmc.dptr(f1), ekmc.dptr(f2)))
+        out += [f, f1, f2]
+    if ac:
+        ph, nc = num.n_integration_phases, num.n_energy_cells
+        per, me, fv = np.zeros(ph * nc), np.zeros(ph), np.zeros(3 * ph)
+        ekmc.check(L.ekmc_get_eedf_periodic(eng.h, c, ekmc.dptr(per), ekmc.dptr(me), ekmc.dptr(fv)))
+        out.append(per)
+    return out
+
+
+def test_histograms_counted_at_load_equal_histogram_kernel(monkeypatch):
+    """The flight kernel counts each requested histogram as it loads the
+    sampled state; the same seeded runs with every histogram from the
+    histogram kernel give identical EEDF, EVDF, anisotropies and per-phase
+    EEDF (cylindrical DC condition and an AC condition)."""
+    mix, _, _ = ekmc.workload("cfg1_atomic40_dc")
+    conds = [condition(en=100.0, n_e=6000), condition(en=150.0, angle=60.0, freq=2e9, n_e=6000)]
+    res = []
+    for fused in ("1", "0"):
+        monkeypatch.setenv("EKMC_FUSED_HIST", fused)
+        num = _short_numerics(41)
+        num.n_integration_phases = 8
+        num.n_integration_points = 300
+        eng = _engine(mix, conds, num)
+        eng.solve()
+        res.append(_distributions(eng, 0, num, True, False) + _distributions(eng, 1, num, False, True))
+        eng.close()
+    assert len(res[0]) == len(res[1])
+    for a, b in zip(*res):
+        assert np.array_equal(a, b)
+    assert res[0][0].sum() > 0
+
+
 def _combined_sigma(a, ea, b, eb, k=4.0):
     return abs(a - b) <= k * np.hypot(ea, eb)
 
