@@ -32,23 +32,31 @@ enum StatusBits : uint32_t {
     ST_NONFINITE = 16u,        // an electron clock or energy became NaN/inf
 };
 
-// Per-process constants, device resident (length P each).
+// Constants of one process, one 64-byte record per process: a collision
+// reads the record of the process it chose (one pointer for the kernels
+// instead of one per field).
+struct ProcRec {
+    int32_t type;        // 0 conservative, 1 ionization, 2 attachment
+    int32_t scat;        // angular scattering model
+    double params[2];    // scattering parameters
+    double energyLoss;   // [eV], negative for superelastic
+    double tMass;        // target mass [kg]
+    double rMass;        // reduced mass [kg]
+    double thermalStd;   // sqrt(k Tg / M) [m/s]
+    double wParam;       // [eV]
+};
+static_assert(sizeof(ProcRec) == 64, "process record is 64 bytes");
+
+// Per-process constants, device resident.
 struct DevProcesses {
     int P, G;
     int gasTempEffect;       // EKMC_GASTEMP_*
     int ionizOp;             // EKMC_IONIZ_*
     double sharingFactor;    // 0.5 equalSharing, 0 oneTakesAll (BoltzmannMC.h:262-270)
-    const int* type;
-    const int* scat;
-    const double* params;      // [2P]
-    const double* energyLoss;  // [eV], negative for superelastic
-    const double* tMass;       // [kg]
-    const double* rMass;       // reduced mass [kg]
-    const double* thermalStd;  // sqrt(k Tg / M) [m/s]
-    const double* wParam;      // [eV]
-    const int* gasFirst;       // [G]
-    const int* gasLast;        // [G]
-    const double* gasFrac;     // [G]
+    const ProcRec* proc;     // [P]
+    const int* gasFirst;     // [G]
+    const int* gasLast;      // [G]
+    const double* gasFrac;   // [G]
 };
 
 // One working condition as the kernels see it.

@@ -333,10 +333,10 @@ __device__ inline void flight(const CondParams& c, Electron<float>& e, double dt
 template <typename V>
 __device__ __forceinline__ V cos_chi(const DevProcesses& pr, int p, V energy, V energyAfter, EventRng<V>& rng,
                                     int slot) {
-    switch (pr.scat[p]) {
+    switch (pr.proc[p].scat) {
         case 1: return (V)1;  // forward
         case 2: {             // bornDipole
-            V loss = (V)pr.energyLoss[p];
+            V loss = (V)pr.proc[p].energyLoss;
             V s = sqrt(energyAfter) + sqrt(energy);
             V ratio = loss / (s * s);
             V r2 = ratio * ratio;
@@ -345,8 +345,8 @@ __device__ __forceinline__ V cos_chi(const DevProcesses& pr, int p, V energy, V 
         case 3:  // surendra
             return ((V)2 + energy - (V)2 * pow((V)1 + energy, rng.u(slot))) / energy;
         case 4: {  // coulombScreen
-            V eps = pr.params[2 * p] == 0 ? energy : energyAfter;
-            V sp = (V)pr.params[2 * p + 1] / eps;
+            V eps = pr.proc[p].params[0] == 0 ? energy : energyAfter;
+            V sp = (V)pr.proc[p].params[1] / eps;
             V R = rng.u(slot);
             return (sp + (V)1 - ((V)2 * sp + (V)1) * R) / (sp + (V)1 - R);
         }
@@ -512,11 +512,11 @@ __device__ __forceinline__ int thermal_choose(const CondParams& c, const DevProc
     for (int g = 0; g < pr.G; ++g) {
         if (pr.gasFrac[g] == 0) continue;
         int L = pr.gasFirst[g], R = pr.gasLast[g];
-        V sd = (V)pr.thermalStd[L];
+        V sd = (V)pr.proc[L].thermalStd;
         for (int k = 0; k < 3; ++k) Vt[k] = vers[k] * sd;
         V dx = e.vx - Vt[0], dy = e.vy - Vt[1], dz = e.vz - Vt[2];
         V rel = sqrt(sqr_norm(dx, dy, dz));
-        V over = (V)0.5 * (V)pr.rMass[L] * rel * rel / (V)kCharge / step;
+        V over = (V)0.5 * (V)pr.proc[L].rMass * rel * rel / (V)kCharge / step;
         int k1 = (int)fmin(over, (V)(I - 1));
         int k2 = (int)fmin((V)(k1 + 1), (V)(I - 1));
         V w1 = (V)k2 - over;
@@ -540,7 +540,8 @@ template <typename V, bool THERMAL>
 __device__ __forceinline__ void apply_collision(const DevProcesses& pr, Electron<V>& e, EventRng<V>& rng,
                                                 CollOut<V>& o, int chosen, const V Vt[3], bool thermal) {
     o.pid = chosen;
-    const int type = pr.type[chosen];
+    const ProcRec& pc = pr.proc[chosen];
+    const int type = pc.type;
     const V inc = e.eps;
     if (type == 2) {
         o.dE = -(double)inc;  // attachmentCollision: the electron is lost
@@ -548,7 +549,7 @@ __device__ __forceinline__ void apply_collision(const DevProcesses& pr, Electron
     }
     if (THERMAL && thermal && type == 0) {
         // conservativeCollision with the target's thermal velocity
-        double M = pr.tMass[chosen], mu = pr.rMass[chosen], loss = pr.energyLoss[chosen];
+        double M = pc.tMass, mu = pc.rMass, loss = pc.energyLoss;
         Sph<V> s;
         if (!cart2sph(e.vx - Vt[0], e.vy - Vt[1], e.vz - Vt[2], s)) { o.flags |= ST_ZERO_VELOCITY; o.pid = -1; return; }
         double relE = 0.5 * mu * (double)s.norm * (double)s.norm / kCharge;
@@ -574,7 +575,7 @@ __device__ __forceinline__ void apply_collision(const DevProcesses& pr, Electron
     // conservativeCollision (cold) and ionizationCollision share the
     // scattering of the incident electron: one sampler, azimuth and rotation
     const bool ion = type == 1;
-    const double loss = pr.energyLoss[chosen];  // energy loss, or the ionization threshold
+    const double loss = pc.energyLoss;  // energy loss, or the ionization threshold
     if (ion && (double)inc < loss) { o.pid = -1; return; }
     Sph<V> s;
     if (!cart2sph(e.vx, e.vy, e.vz, s)) { o.flags |= ST_ZERO_VELOCITY; o.pid = -1; return; }
@@ -584,7 +585,7 @@ __device__ __forceinline__ void apply_collision(const DevProcesses& pr, Electron
     V ej = (V)0, scE = net;
     if (ion) {
         if (pr.ionizOp == 2) {
-            V w = (V)pr.wParam[chosen];
+            V w = (V)pc.wParam;
             ej = w * tan(rng.u(8) * atan(net / ((V)2 * w)));
         } else if (pr.ionizOp == 3) {
             ej = rng.u(8) * net;
@@ -593,7 +594,7 @@ __device__ __forceinline__ void apply_collision(const DevProcesses& pr, Electron
         }
         scE = net - ej;
     }
-    const bool boeuf = ion && pr.scat[chosen] == 5;
+    const bool boeuf = ion && pc.scat == 5;
     V cc;
     if (boeuf) cc = sqrt(scE / net);
     else cc = cos_chi<V>(pr, chosen, inc, scE, rng, 6);
@@ -606,7 +607,7 @@ __device__ __forceinline__ void apply_collision(const DevProcesses& pr, Electron
     if (ion) {
         sp = sqrt((V)2 * scE * (V)kCharge / (V)kMass);
     } else {
-        const double M = pr.tMass[chosen], mu = pr.rMass[chosen];
+        const double M = pc.tMass, mu = pc.rMass;
         double v2after = (double)s.norm * (double)s.norm - 2.0 / kMass * loss * kCharge;
         if (!(v2after > 0)) { o.pid = -1; return; }  // rounding at the threshold edge
         sp = sqrt((V)v2after * ((V)1 - (V)(2.0 * mu / (kMass + M)) * ((V)1 - cc)));

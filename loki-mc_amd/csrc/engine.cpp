@@ -233,6 +233,7 @@ public:
     std::vector<int32_t> listActive;  // active flags the flight lists were built for
     bool histFlagsOnDevice = false;   // the device grid holds fill flags of the last interval
     bool fusedHist = true;            // running conditions' histograms counted by the flight kernel
+    DevBuf<ProcRec> prec;
     DevBuf<int> iproc;
     DevBuf<double> dproc;
     DevBuf<unsigned long long> hEedf, hPer, hEadf, hEvdf;
@@ -264,7 +265,7 @@ public:
         surplusC.release(); markC.release(); flist.release(); phaseRow.release(); cylFlag.release();
         holeOff.release(); surOff.release(); markOff.release(); mainBase.release(); tailC.release();
         momC.release(); summary.release(); thermalDev.release(); grid.release(); accCount.release();
-        accGain.release(); accLoss.release(); plans.release(); iproc.release(); dproc.release();
+        accGain.release(); accLoss.release(); plans.release(); prec.release(); iproc.release(); dproc.release();
         hEedf.release(); hPer.release(); hEadf.release(); hEvdf.release(); ranges.release();
         hsum.release(); pinCond.release(); pinNumax.release(); tsrcD.release(); tsrcI.release();
         nutot64.release(); numax64.release();
@@ -440,33 +441,40 @@ public:
         HIP_TRY(status.alloc(n));
         HIP_TRY(hipMemsetAsync(status.p, 0, n * sizeof(uint32_t), stream));
         HIP_TRY(hipMemsetAsync(plans.p, 0, n * sizeof(PopPlan), stream));
-        // processes
-        std::vector<int> hi;
-        std::vector<double> hd;
-        auto pushI = [&](const std::vector<int>& v) { size_t o = hi.size(); hi.insert(hi.end(), v.begin(), v.end()); return o; };
-        auto pushD = [&](const std::vector<double>& v) { size_t o = hd.size(); hd.insert(hd.end(), v.begin(), v.end()); return o; };
-        std::vector<double> thermalStd(P);
-        for (int p = 0; p < P; ++p) thermalStd[p] = 0;  // set per condition temperature below
-        size_t oType = pushI(pl.type), oScat = pushI(pl.scat), oFirst = pushI(pl.gasFirst), oLast = pushI(pl.gasLast);
-        size_t oPar = pushD(pl.params), oLoss = pushD(pl.energyLoss), oM = pushD(pl.tMass), oMu = pushD(pl.rMass);
+        // processes: one record per process, plus the per-gas ranges
         // thermal standard deviation uses the gas temperature, shared by all conditions
         double Tg0 = cs[0].Tg;
         for (int c = 1; c < n; ++c)
             if (cs[c].Tg != Tg0 && num.gas_temperature_effect != EKMC_GASTEMP_FALSE)
                 return fail(EKMC_ERR_INVALID_ARGUMENT, "batched conditions must share the gas temperature when thermal motion is on");
-        for (int p = 0; p < P; ++p) thermalStd[p] = std::sqrt(kBoltzmann * Tg0 / pl.tMass[p]);
-        size_t oStd = pushD(thermalStd), oW = pushD(pl.wParam), oFrac = pushD(pl.gasFrac);
+        std::vector<ProcRec> hrec(P);
+        for (int p = 0; p < P; ++p) {
+            ProcRec& r = hrec[p];
+            r.type = pl.type[p];
+            r.scat = pl.scat[p];
+            r.params[0] = pl.params[2 * p];
+            r.params[1] = pl.params[2 * p + 1];
+            r.energyLoss = pl.energyLoss[p];
+            r.tMass = pl.tMass[p];
+            r.rMass = pl.rMass[p];
+            r.thermalStd = std::sqrt(kBoltzmann * Tg0 / pl.tMass[p]);
+            r.wParam = pl.wParam[p];
+        }
+        std::vector<int> hi(pl.gasFirst);
+        hi.insert(hi.end(), pl.gasLast.begin(), pl.gasLast.end());
+        HIP_TRY(prec.alloc(P));
         HIP_TRY(iproc.alloc(hi.size()));
-        HIP_TRY(dproc.alloc(hd.size()));
+        HIP_TRY(dproc.alloc(pl.gasFrac.size()));
+        HIP_TRY(hipMemcpyAsync(prec.p, hrec.data(), P * sizeof(ProcRec), hipMemcpyHostToDevice, stream));
         HIP_TRY(hipMemcpyAsync(iproc.p, hi.data(), hi.size() * sizeof(int), hipMemcpyHostToDevice, stream));
-        HIP_TRY(hipMemcpyAsync(dproc.p, hd.data(), hd.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(dproc.p, pl.gasFrac.data(), pl.gasFrac.size() * sizeof(double), hipMemcpyHostToDevice,
+                               stream));
         dpr.P = P; dpr.G = pl.G;
         dpr.gasTempEffect = num.gas_temperature_effect;
         dpr.ionizOp = num.ionization_operator;
         dpr.sharingFactor = num.ionization_operator == EKMC_IONIZ_ONE_TAKES_ALL ? 0.0 : 0.5;
-        dpr.type = iproc.p + oType; dpr.scat = iproc.p + oScat; dpr.gasFirst = iproc.p + oFirst; dpr.gasLast = iproc.p + oLast;
-        dpr.params = dproc.p + oPar; dpr.energyLoss = dproc.p + oLoss; dpr.tMass = dproc.p + oM; dpr.rMass = dproc.p + oMu;
-        dpr.thermalStd = dproc.p + oStd; dpr.wParam = dproc.p + oW; dpr.gasFrac = dproc.p + oFrac;
+        dpr.proc = prec.p;
+        dpr.gasFirst = iproc.p; dpr.gasLast = iproc.p + pl.G; dpr.gasFrac = dproc.p;
         // collision-table sources: prepared cross sections and interpolation constants
         {
             std::vector<double> td, te, ts;

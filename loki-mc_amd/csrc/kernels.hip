@@ -424,7 +424,7 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
                     const long long fx = __double2ll_rn(o.dE * kFix);
                     if (o.dE >= 0) atomicAdd((unsigned long long*)&gainL[o.pid], (unsigned long long)fx);
                     else atomicAdd((unsigned long long*)&lossL[o.pid], (unsigned long long)fx);
-                    const int type = pr.type[o.pid];
+                    const int type = pr.proc[o.pid].type;
                     if (type == 1) spawn = true;
                     else if (type == 2) { done = true; dead = true; }
                 }
