@@ -56,10 +56,11 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kParkMin = 32;
 
 // LDS of one wave of the flight kernel: collision tallies [3P] (64-bit), the
-// null-test envelope of the chunk's condition [2 kEnvBins] (V) and the EEDF
-// counts of a histogram taken at load [histLds] (32-bit).
+// histogram grid of the chunk's condition [8] (FP64), the null-test envelope
+// [2 kEnvBins] (V) and the EEDF counts of a histogram taken at load [histLds]
+// (32-bit).  The record ring is a separate static array.
 template <typename V> __host__ __device__ constexpr size_t flight_wave_lds(int P, int histLds) {
-    return (size_t)3 * P * 8 + (size_t)2 * kEnvBins * sizeof(V) + (size_t)histLds * 4;
+    return (size_t)3 * P * 8 + 8 * 8 + (size_t)2 * kEnvBins * sizeof(V) + (size_t)histLds * 4;
 }
 
 // ------------------------------------------------------------ wave helpers
@@ -122,6 +123,83 @@ template <typename V> __device__ __forceinline__ void store_rec(Rec<V>* r, int64
 }
 template <typename V> __device__ __forceinline__ double rec_energy(const Rec<V>& q) {
     return (double)energy_of(q.vx, q.vy, q.vz);
+}
+
+// ------------------------------------------------------------ record ring
+// The main records of a chunk stream through a ring of 2 x kRingHalf records
+// per wave in LDS, filled by direct-to-LDS loads issued at least half a ring
+// ahead of use: a lane taking the next main electron reads LDS, and the HBM
+// latency of the record overlaps the flights of the wave.  Chunk-relative
+// main record i lives in half (i / kRingHalf) % 2, slot i % kRingHalf; piece k
+// (16 bytes) of slot j is half[k * kRingHalf + j].  Lane j < kRingHalf issues
+// the loads of slot j (the LDS address a lane writes is the base plus
+// 16 * lane).  The wave orders the ring itself: the first read of a half
+// after its fill waits for every outstanding vector memory operation
+// (vmcnt(0)), and a read completes inside its own instruction group, so a
+// later fill of that half cannot overtake it.
+constexpr int kRingHalf = kWave / 2;
+template <typename V> __host__ __device__ constexpr int rec_pieces() { return (int)(sizeof(Rec<V>) / 16); }
+typedef __attribute__((address_space(1))) void global_void;
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(3))) uint4 lds_uint4;  // LDS pointers stay LDS pointers
+
+// Loads of main records [loaded, loaded + kRingHalf) of a chunk of `count`
+// into `half`.
+template <typename V>
+__device__ __forceinline__ void ring_fill(const Rec<V>* rec, int64_t first, int count, int loaded, lds_uint4* half) {
+    const int lane = lane_id();
+    const int idx = loaded + lane;
+    if (lane < kRingHalf && idx < count) {
+        const char* src = (const char*)(rec + first + idx);
+#pragma unroll
+        for (int k = 0; k < rec_pieces<V>(); ++k)
+            __builtin_amdgcn_global_load_lds((global_void*)(src + 16 * k), (lds_void*)(half + k * kRingHalf), 16, 0,
+                                             0);
+    }
+}
+// The pieces of one ring slot at LDS byte address `a`, read and waited for
+// in one group.
+__device__ __forceinline__ void ring_pieces(uint32_t a, uint4 (&p)[4]) {
+    asm volatile(
+        "ds_read_b128 %0, %4\n\t"
+        "ds_read_b128 %1, %4 offset:512\n\t"
+        "ds_read_b128 %2, %4 offset:1024\n\t"
+        "ds_read_b128 %3, %4 offset:1536\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(p[0]), "=&v"(p[1]), "=&v"(p[2]), "=&v"(p[3])
+        : "v"(a)
+        : "memory");
+}
+__device__ __forceinline__ void ring_pieces(uint32_t a, uint4 (&p)[6]) {
+    asm volatile(
+        "ds_read_b128 %0, %6\n\t"
+        "ds_read_b128 %1, %6 offset:512\n\t"
+        "ds_read_b128 %2, %6 offset:1024\n\t"
+        "ds_read_b128 %3, %6 offset:1536\n\t"
+        "ds_read_b128 %4, %6 offset:2048\n\t"
+        "ds_read_b128 %5, %6 offset:2560\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(p[0]), "=&v"(p[1]), "=&v"(p[2]), "=&v"(p[3]), "=&v"(p[4]), "=&v"(p[5])
+        : "v"(a)
+        : "memory");
+}
+static_assert(kRingHalf * 16 == 512, "ring piece stride");
+// LDS atomic add without return, issued as is: the compiler would otherwise
+// order it after the outstanding ring loads (it cannot tell the histogram
+// from the ring), which would expose their HBM latency.
+__device__ __forceinline__ void lds_add_u32(uint32_t* p, uint32_t v) {
+    asm volatile("ds_add_u32 %0, %1" : : "v"((uint32_t)(size_t)(lds_void*)p), "v"(v) : "memory");
+}
+// Slot j of a ring half.
+template <typename V> __device__ __forceinline__ void ring_read(const lds_uint4* half, int j, Electron<V>& e) {
+    uint4 p[rec_pieces<V>()];
+    ring_pieces((uint32_t)(size_t)(half + j), p);
+    Rec<V> q;
+    __builtin_memcpy(&q, p, sizeof(q));
+    e.x = q.x; e.y = q.y; e.z = q.z; e.t = q.t;
+    e.vx = q.vx; e.vy = q.vy; e.vz = q.vz;
+    e.eps = energy_of(q.vx, q.vy, q.vz);
+    e.tfree = q.tfree; e.nue = q.nue; e.stream = q.stream; e.event = q.event;
 }
 
 // ------------------------------------------------------- block helpers
@@ -281,21 +359,21 @@ __device__ void wave_refill(const DevState<V>& s, uint32_t* holeList, const Chun
 // One electron into the histograms of condition c (getTimeDependDistributions,
 // BoltzmannMC.C:1551-1571; MathFunctions::histogramCount/histogram2DCount,
 // MathFunctions.C:61-81,106-127): the EEDF into `lhist` (LDS, flushed later)
-// or directly into HBM, the cylindrical EADF and EVDF into HBM.
+// or directly into HBM (phase row `row`, or -1), the EADF and EVDF into HBM
+// when `cyl`.  The per-condition values are read once per chunk by the caller.
 template <typename V>
 __device__ __forceinline__ void hist_count(const DevHist& h, int c, const double* g, const Electron<V>& e,
-                                           uint32_t* lhist) {
+                                           uint32_t* lhist, int row, bool cyl) {
     const double en = (double)e.eps;
     const int cell = (int)(en / g[0]);
     if (cell >= 0 && cell < h.nCells) {
-        if (lhist) atomicAdd(&lhist[cell], 1u);
+        if (lhist) lds_add_u32(lhist + cell, 1u);
         else {
-            const int row = h.phaseRow[c];
             atomicAdd(&h.eedf[(size_t)c * h.nCells + cell], 1ull);
             if (row >= 0 && h.eedfPer) atomicAdd(&h.eedfPer[((size_t)c * h.nPhases + row) * h.nCells + cell], 1ull);
         }
     }
-    if (h.cyl[c] != 0 && h.eadf) {
+    if (cyl) {
         const double vx = e.vx, vy = e.vy, vz = e.vz;
         const double cosA = vz / sqrt(vx * vx + vy * vy + vz * vz);
         const int cy = (int)((cosA + 1.0) / g[1]);
@@ -313,7 +391,8 @@ __device__ __forceinline__ void hist_count(const DevHist& h, int c, const double
 // passes of BoltzmannMC.C:630-681 for this set of electrons).
 template <typename V, int FC, bool THERMAL>
 __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWork& w, const DevProcesses& pr,
-                                              const V* tables, const DevHist& h, int cid, unsigned long long* cnt) {
+                                              const V* tables, const DevHist& h, int cid, unsigned long long* cnt,
+                                              lds_uint4* ringA, lds_uint4* ringB) {
     const int lane = lane_id();
     const int P = pr.P;
     long long* gainL = (long long*)(cnt + P);
@@ -325,22 +404,32 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
     const V* cumul = tables + cp.tableOffset;
     const V* interp = cumul + I * P;
     const V* nutot = interp + I * P;
-    V* envHi = (V*)(cnt + 3 * P);
+    double* hg = (double*)(cnt + 3 * P);
+    V* envHi = (V*)(hg + 8);
     V* envLo = envHi + kEnvBins;
     uint32_t* lhist = (uint32_t*)(envLo + kEnvBins);
     for (int i = lane; i < 3 * P; i += kWave) ((long long*)cnt)[i] = 0;
     for (int i = lane; i < 2 * kEnvBins; i += kWave) envHi[i] = nutot[I + i];
     // histograms of the sampled state (the state this interval starts from),
-    // counted as main electrons are loaded (fill flag 2 of the condition)
-    const double* hg = h.grid + 6 * (size_t)c;
-    const bool histOn = hg[5] == 2.0;
+    // counted as main electrons are loaded (fill flag 2 of the condition);
+    // the condition's grid is copied to LDS
+    if (lane < 6) hg[lane] = h.grid[6 * (size_t)c + lane];
+    const bool histOn = h.grid[6 * (size_t)c + 5] == 2.0;
     const bool histLds = histOn && h.histLds >= h.nCells;
+    const int histRow = histOn ? h.phaseRow[c] : -1;
+    const bool histCyl = histOn && h.eadf && h.cyl[c] != 0;
     if (histLds)
         for (int i = lane; i < h.nCells; i += kWave) lhist[i] = 0;
     wave_lds_sync();
 
     const int count = ch.count;
     int nextMain = 0, nq = 0, nextNew = 0, holes = 0;
+    int loaded = 0;  // main records whose ring loads are issued
+    bool pendA = false, pendB = false;  // a half filled since the last vmcnt(0) of the wave
+    for (; loaded < count && loaded <= nextMain + kRingHalf; loaded += kRingHalf) {
+        if ((loaded / kRingHalf) & 1) { ring_fill<V>(s.rec, ch.first, count, loaded, ringB); pendB = true; }
+        else { ring_fill<V>(s.rec, ch.first, count, loaded, ringA); pendA = true; }
+    }
     Electron<V> e;
     bool have = false, isNew = false, parked = false;
     int64_t slot = 0;
@@ -353,10 +442,20 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
         const uint64_t need = __ballot(!have);
         if (need) {
             const int nNeed = __popcll(need);
-            const int availMain = count - nextMain;
+            const int availMain = (loaded < count ? loaded : count) - nextMain;
             const int takeMain = availMain < nNeed ? availMain : nNeed;
             const int availNew = nq - nextNew;
             const int takeNew = (nNeed - takeMain) < availNew ? (nNeed - takeMain) : availNew;
+            if (takeMain > 0) {
+                // main records [nextMain, nextMain + takeMain) span at most two halves
+                const int h0 = (nextMain / kRingHalf) & 1, h1 = ((nextMain + takeMain - 1) / kRingHalf) & 1;
+                const bool useA = h0 == 0 || h1 == 0 || takeMain > kRingHalf;
+                const bool useB = h0 == 1 || h1 == 1 || takeMain > kRingHalf;
+                if ((useA && pendA) || (useB && pendB)) {
+                    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): the ring loads have landed
+                    pendA = pendB = false;
+                }
+            }
             if (!have) {
                 const int r = lane_rank(need);
                 if (r < takeMain) {
@@ -369,9 +468,15 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
                     have = true;
                 }
                 if (have) {
-                    load_rec(s.rec, slot, e);
+                    if (isNew) {
+                        load_rec(s.rec, slot, e);
+                    } else {
+                        const int idx = (int)(slot - ch.first);
+                        if ((idx / kRingHalf) & 1) ring_read<V>(ringB, idx & (kRingHalf - 1), e);
+                        else ring_read<V>(ringA, idx & (kRingHalf - 1), e);
+                    }
                     iter = 0;
-                    if (histOn && !isNew) hist_count<V>(h, c, hg, e, histLds ? lhist : nullptr);
+                    if (histOn && !isNew) hist_count<V>(h, c, hg, e, histLds ? lhist : nullptr, histRow, histCyl);
                 }
             }
             nextMain += takeMain;
@@ -468,6 +573,12 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
             have = false;
         }
         holes += __popcll(dm);
+        // refill a consumed ring half, after this step's memory waits (a later
+        // wait on a vector load covers the loads issued before it)
+        for (; loaded < count && loaded <= nextMain + kRingHalf; loaded += kRingHalf) {
+            if ((loaded / kRingHalf) & 1) { ring_fill<V>(s.rec, ch.first, count, loaded, ringB); pendB = true; }
+            else { ring_fill<V>(s.rec, ch.first, count, loaded, ringA); pendA = true; }
+        }
     }
 
     int remaining = 0;
@@ -521,14 +632,18 @@ __global__ __launch_bounds__(kBlock) void k_advance(DevState<V> s, DevWork w, De
                                                     DevHist h, const int32_t* list, int32_t nList,
                                                     uint32_t* queue) {
     extern __shared__ unsigned char smem[];
+    __shared__ uint4 ringsA[kWavesPerBlock * rec_pieces<V>() * kRingHalf];
+    __shared__ uint4 ringsB[kWavesPerBlock * rec_pieces<V>() * kRingHalf];
     const int wid = threadIdx.x / kWave;
     unsigned long long* cnt = (unsigned long long*)(smem + (size_t)wid * flight_wave_lds<V>(pr.P, h.histLds));
+    lds_uint4* ringA = (lds_uint4*)(ringsA + wid * rec_pieces<V>() * kRingHalf);
+    lds_uint4* ringB = (lds_uint4*)(ringsB + wid * rec_pieces<V>() * kRingHalf);
     while (true) {
         uint32_t item = 0;
         if (lane_id() == 0) item = atomicAdd(queue, 1u);
         item = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl((int)item, 0, kWave));
         if (item >= (uint32_t)nList) break;
-        advance_chunk<V, FC, THERMAL>(s, w, pr, tables, h, list[item], cnt);
+        advance_chunk<V, FC, THERMAL>(s, w, pr, tables, h, list[item], cnt, ringA, ringB);
     }
 }
 
