@@ -772,16 +772,35 @@ __global__ __launch_bounds__(1024) void k_plan_marks(DevWork w) {
             if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t need = sh[2], cum = 0;
-            int bkt = 0;
-            for (; bkt < 255; ++bkt) {
-                if (cum + hist[bkt] >= need) break;
-                cum += hist[bkt];
+        if (threadIdx.x < kWave) {
+            // the bucket holding the need-th smallest key: lane l scans
+            // buckets [4l, 4l + 4) after an exclusive scan of the lane sums
+            // (bucket 255 takes the rest, as a serial scan would)
+            const int l = threadIdx.x;
+            const uint32_t need = sh[2];
+            uint32_t b[4], sum = 0;
+            for (int q = 0; q < 4; ++q) { b[q] = hist[4 * l + q]; sum += b[q]; }
+            uint32_t incl = sum;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint32_t t = __shfl_up(incl, o, kWave);
+                if (l >= o) incl += t;
             }
-            sh[0] = prefix | ((uint32_t)bkt << shift);
-            sh[1] = mask | (255u << shift);
-            sh[2] = need - cum;
+            uint32_t cum = incl - sum;
+            int found = -1;
+            uint32_t before = 0;
+            for (int q = 0; q < 4; ++q) {
+                const int bkt = 4 * l + q;
+                if (found < 0 && (cum + b[q] >= need || bkt == 255)) { found = bkt; before = cum; }
+                cum += b[q];
+            }
+            // the first lane with a bucket is the answer
+            const uint64_t has = __ballot(found >= 0);
+            const int first = __ffsll((long long)has) - 1;
+            if (l == first) {
+                sh[0] = prefix | ((uint32_t)found << shift);
+                sh[1] = mask | (255u << shift);
+                sh[2] = need - before;
+            }
         }
         __syncthreads();
     }
@@ -990,8 +1009,7 @@ __global__ __launch_bounds__(kBlock) void k_sample(DevState<V> s, DevWork w) {
     if (!w.conds[ch.cond].active) return;
     double m[kMomentRecord];
     for (int k = 0; k < kMomentRecord; ++k) m[k] = 0;
-    for (int64_t k = lane; k < ch.count; k += kWave) {
-        const Rec<V> q = s.rec[ch.first + k];
+    auto add = [&](const Rec<V>& q) {
         const double r[3] = {q.x, q.y, q.z};
         const double v[3] = {(double)q.vx, (double)q.vy, (double)q.vz};
         const double e = rec_energy(q);
@@ -1001,7 +1019,16 @@ __global__ __launch_bounds__(kBlock) void k_sample(DevState<V> s, DevWork w) {
             for (int bb = 0; bb < 3; ++bb) { m[7 + 3 * a + bb] += r[a] * r[bb]; m[16 + 3 * a + bb] += r[a] * v[bb]; }
         m[25] += 1;
         m[26] = fmax(m[26], e);
+    };
+    // two records per lane in flight; each lane still adds its records in
+    // slot order
+    int64_t k = lane;
+    for (; k + kWave < ch.count; k += 2 * kWave) {
+        const Rec<V> q0 = s.rec[ch.first + k], q1 = s.rec[ch.first + k + kWave];
+        add(q0);
+        add(q1);
     }
+    if (k < ch.count) add(s.rec[ch.first + k]);
     for (int k = 0; k < kMomentRecord - 1; ++k) m[k] = wave_sum(m[k]);
     m[26] = wave_max(m[26]);
     if (lane == 0)
