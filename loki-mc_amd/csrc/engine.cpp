@@ -231,7 +231,11 @@ public:
     DevBuf<PopPlan> plans;
     FlightLists fl = {};
     std::vector<int32_t> listActive;  // active flags the flight lists were built for
-    bool histFlagsOnDevice = false;   // the device grid holds fill flags of the last interval
+    // histogram grid and phase rows as last uploaded (staged in pinned memory)
+    PinnedBuf<double> pinGrid;
+    PinnedBuf<int32_t> pinRow;
+    std::vector<double> upGrid;
+    std::vector<int32_t> upRow;
     bool fusedHist = true;            // running conditions' histograms counted by the flight kernel
     DevBuf<ProcRec> prec;
     DevBuf<int> iproc;
@@ -268,6 +272,7 @@ public:
         accGain.release(); accLoss.release(); plans.release(); prec.release(); iproc.release(); dproc.release();
         hEedf.release(); hPer.release(); hEadf.release(); hEvdf.release(); ranges.release();
         hsum.release(); pinCond.release(); pinNumax.release(); tsrcD.release(); tsrcI.release();
+        pinGrid.release(); pinRow.release();
         nutot64.release(); numax64.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -552,6 +557,8 @@ public:
         HIP_TRY(hsum.alloc((size_t)n * summarySize));
         std::memset(hsum.p, 0, hsum.n * sizeof(double));
         HIP_TRY(pinCond.alloc(n));
+        HIP_TRY(pinGrid.alloc((size_t)n * 6));
+        HIP_TRY(pinRow.alloc(n));
         // histogram blocks: ranges of at most 8192 main slots of one condition
         std::vector<SlotRange> hr;
         for (int c = 0; c < n; ++c)
@@ -611,6 +618,20 @@ public:
             int k = (int)std::fmin(std::ceil(maxEnergy / C.tab.step), I - 1);
             if (C.trial < C.tab.numax[k]) { updated = true; C.trial *= 1.1; }
         }
+        return EKMC_OK;
+    }
+
+    // The histogram grid and phase rows for the next kernels, uploaded only
+    // when they differ from the last upload (the stream was synchronized
+    // since that upload, so the pinned staging may be rewritten).
+    ekmc_status upload_hist_grid(const std::vector<double>& g, const std::vector<int32_t>& r) {
+        if (g == upGrid && r == upRow) return EKMC_OK;
+        std::memcpy(pinGrid.p, g.data(), g.size() * sizeof(double));
+        std::memcpy(pinRow.p, r.data(), r.size() * sizeof(int32_t));
+        HIP_TRY(hipMemcpyAsync(grid.p, pinGrid.p, g.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(phaseRow.p, pinRow.p, r.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        upGrid = g;
+        upRow = r;
         return EKMC_OK;
     }
 
@@ -952,11 +973,10 @@ public:
                 hcp[c].syncIndex = C.syncIndex;
             }
             if (active == 0 && !anyHist) break;
-            if (anyHist || histFlagsOnDevice) {
+            {
                 // histograms of the state sampled at the end of the previous interval
-                HIP_TRY(hipMemcpyAsync(grid.p, hgrid.data(), hgrid.size() * sizeof(double), hipMemcpyHostToDevice, stream));
-                HIP_TRY(hipMemcpyAsync(phaseRow.p, hrow.data(), nCond * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-                histFlagsOnDevice = anyHist;
+                ekmc_status st = upload_hist_grid(hgrid, hrow);
+                if (st != EKMC_OK) return st;
                 if (anyHistKernel) HIP_TRY(launch_hist<V>(ds, dh, stream));
             }
             if (active == 0) { HIP_TRY(hipStreamSynchronize(stream)); break; }
@@ -1068,10 +1088,9 @@ public:
             gr[0] = C.eedfStep; gr[1] = C.cosStep; gr[2] = C.radStep; gr[3] = C.axStep; gr[4] = C.maxSpeed;
             if (C.histRequest) { gr[5] = 1; any = true; hrow[c] = C.histPhase; C.histRequest = 0; }
         }
-        if (any || histFlagsOnDevice) {
-            HIP_TRY(hipMemcpyAsync(grid.p, hgrid.data(), hgrid.size() * sizeof(double), hipMemcpyHostToDevice, stream));
-            HIP_TRY(hipMemcpyAsync(phaseRow.p, hrow.data(), nCond * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-            histFlagsOnDevice = any;
+        {
+            ekmc_status st = upload_hist_grid(hgrid, hrow);
+            if (st != EKMC_OK) return st;
             if (any) HIP_TRY(launch_hist<V>(ds, dh, stream));
         }
         HIP_TRY(hipStreamSynchronize(stream));

@@ -1168,40 +1168,61 @@ __device__ __forceinline__ double table_at(const double* x, const double* y, int
     return dx > 0.0 ? y[lo] + (v - x[lo]) / dx * (y[lo + 1] - y[lo]) : 0.0;
 }
 
-// One table row per thread, in the arithmetic and order of the host FP64
-// build (interpolateCrossSections, BoltzmannMC.C:571-614).
+// Interpolated cross section x relative density of process p at `energy`
+// (interpolateCrossSections, BoltzmannMC.C:571-609), FP64.
+__device__ __forceinline__ double table_value(const DevTableSrc& a, int p, double energy) {
+#pragma clang fp contract(off)
+    const int b = a.srcBegin[p], n = a.srcEnd[p] - b;
+    double value = 0;
+    if (a.isSuper[p]) {
+        // Klein-Rosseland: (g_i/g_f)(1 + E_th/u) sigma_if(u + E_th)
+        if (energy > a.minE[p] && energy <= a.maxE[p]) {
+            const double thr = a.thrPartner[p];
+            value = a.statW[p] * (1.0 + thr / energy) * table_at(a.energy + b, a.sigma + b, n, energy + thr) *
+                    a.relDens[p];
+        }
+    } else if (energy >= a.minE[p] && energy <= a.maxE[p]) {
+        value = table_at(a.energy + b, a.sigma + b, n, energy) * a.relDens[p];
+    }
+    return value;
+}
+
+// Rows of a table block: every (row, process) value by its own thread into
+// LDS, then each row's running sum in process order by one thread -- the
+// arithmetic and order of the host FP64 build (BoltzmannMC.C:571-614).
+// A block holds table_rows_per_block(P) rows.
+inline __host__ __device__ int table_rows_per_block(int P) { return P >= kBlock ? 1 : kBlock / P; }
 template <typename V>
 __global__ __launch_bounds__(kBlock) void k_table_rows(DevTableSrc a, double N, double step, V* tables,
                                                        double* nutot64) {
 #pragma clang fp contract(off)
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= a.I) return;
-    const int P = a.P;
+    extern __shared__ double vals[];  // [rows * P]
+    const int P = a.P, I = a.I;
+    const int rows = table_rows_per_block(P);
+    const int i0 = blockIdx.x * rows;
     V* cumul = tables;
-    V* interp = tables + (size_t)a.I * P;
-    V* nutot = interp + (size_t)a.I * P;
-    const double energy = i * step;
-    double acc = 0;
-    for (int p = 0; p < P; ++p) {
-        const int b = a.srcBegin[p], n = a.srcEnd[p] - b;
-        double value = 0;
-        if (a.isSuper[p]) {
-            // Klein-Rosseland: (g_i/g_f)(1 + E_th/u) sigma_if(u + E_th)
-            if (energy > a.minE[p] && energy <= a.maxE[p]) {
-                const double thr = a.thrPartner[p];
-                value = a.statW[p] * (1.0 + thr / energy) * table_at(a.energy + b, a.sigma + b, n, energy + thr) *
-                        a.relDens[p];
-            }
-        } else if (energy >= a.minE[p] && energy <= a.maxE[p]) {
-            value = table_at(a.energy + b, a.sigma + b, n, energy) * a.relDens[p];
-        }
-        interp[(size_t)i * P + p] = (V)value;
-        acc += value;
-        cumul[(size_t)i * P + p] = (V)acc;
+    V* interp = tables + (size_t)I * P;
+    V* nutot = interp + (size_t)I * P;
+    for (int q = threadIdx.x; q < rows * P; q += kBlock) {
+        const int i = i0 + q / P;
+        if (i >= I) break;
+        const double v = table_value(a, q % P, i * step);
+        vals[q] = v;
+        interp[(size_t)i0 * P + q] = (V)v;
     }
-    acc *= N * sqrt(energy * 2.0 * kCharge / kMass);
-    nutot[i] = (V)acc;
-    nutot64[i] = acc;
+    __syncthreads();
+    for (int r = threadIdx.x; r < rows; r += kBlock) {
+        const int i = i0 + r;
+        if (i >= I) break;
+        double acc = 0;
+        for (int p = 0; p < P; ++p) {
+            acc += vals[r * P + p];
+            cumul[(size_t)i * P + p] = (V)acc;
+        }
+        acc *= N * sqrt(i * step * 2.0 * kCharge / kMass);
+        nutot[i] = (V)acc;
+        nutot64[i] = acc;
+    }
 }
 
 // Running maximum of the total collision frequency (BoltzmannMC.C:612-613):
@@ -1253,7 +1274,9 @@ __global__ __launch_bounds__(kEnvBins) void k_table_env(V* tables, int I, int P,
 template <typename V>
 hipError_t launch_tables(const DevTableSrc& src, double N, double step, int envShift, V* tables, double* nutot64,
                          double* numax64, hipStream_t st) {
-    hipLaunchKernelGGL(k_table_rows<V>, dim3(grid_of(src.I)), dim3(kBlock), 0, st, src, N, step, tables, nutot64);
+    const int rows = table_rows_per_block(src.P);
+    hipLaunchKernelGGL(k_table_rows<V>, dim3((src.I + rows - 1) / rows), dim3(kBlock),
+                       (size_t)rows * src.P * sizeof(double), st, src, N, step, tables, nutot64);
     hipLaunchKernelGGL(k_table_env<V>, dim3(1), dim3(kEnvBins), 0, st, tables, src.I, src.P, envShift);
     hipLaunchKernelGGL(k_table_max, dim3(1), dim3(1024), 0, st, nutot64, numax64, src.I);
     return hipGetLastError();
