@@ -352,7 +352,9 @@ public:
             if (prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;
             if (prop.sharedMemPerBlock > 0) ldsMax = prop.sharedMemPerBlock;
         }
-        if ((size_t)(kBlock / kWave) * (3 * P * 8 + 2 * kEnvBins * sizeof(V) + 4 * std::min(num.n_energy_cells, 2048)) > ldsMax)
+        // per-wave LDS of the flight kernel (tallies, histogram grid, envelope)
+        // and its record ring; the EEDF is counted in LDS only where it fits
+        if ((size_t)(kBlock / kWave) * (3 * P * 8 + 64 + 2 * kEnvBins * sizeof(V)) + flight_ring_lds<V>() > ldsMax)
             return fail(EKMC_ERR_INVALID_ARGUMENT, "too many processes for the per-wave collision tallies in LDS");
         if (num.work_chunk < 0 || num.reserved != 0) return fail(EKMC_ERR_INVALID_ARGUMENT, "invalid work_chunk/reserved");
         int64_t chunkTarget = num.work_chunk > 0 ? num.work_chunk : kChunkDefault;
@@ -540,7 +542,14 @@ public:
         dh.eedf = hEedf.p; dh.eedfPer = hPer.p; dh.eadf = hEadf.p; dh.evdf = hEvdf.p; dh.grid = grid.p;
         dh.phaseRow = phaseRow.p; dh.cyl = cylFlag.p;
         dh.nCells = nCells; dh.nCos = nCos; dh.nRad = nRad; dh.nAx = nAx; dh.nPhases = nPhases;
-        dh.histLds = nCells <= 2048 ? nCells : 0;
+        // the EEDF of a histogram taken at load is counted in LDS when that
+        // keeps the flight kernel's blocks per CU
+        dh.histLds = 0;
+        if (nCells <= 2048) {
+            const bool thermal = num.gas_temperature_effect != EKMC_GASTEMP_FALSE;
+            if (flight_blocks_per_cu<V>(FIELD_DC, thermal, P, nCells) >= flight_blocks_per_cu<V>(FIELD_DC, thermal, P, 0))
+                dh.histLds = nCells;
+        }
         // first main electron of each condition, for the init kernel
         std::vector<int64_t> mb(n);
         std::vector<double> td(n);

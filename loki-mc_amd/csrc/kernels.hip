@@ -138,6 +138,7 @@ template <typename V> __device__ __forceinline__ double rec_energy(const Rec<V>&
 // (vmcnt(0)), and a read completes inside its own instruction group, so a
 // later fill of that half cannot overtake it.
 constexpr int kRingHalf = kWave / 2;
+static_assert(flight_ring_lds<float>() == 2 * kWavesPerBlock * kRingHalf * sizeof(Rec<float>), "ring size");
 template <typename V> __host__ __device__ constexpr int rec_pieces() { return (int)(sizeof(Rec<V>) / 16); }
 typedef __attribute__((address_space(1))) void global_void;
 typedef __attribute__((address_space(3))) void lds_void;
