@@ -428,3 +428,57 @@ def test_mixed_precision_matches_fp64_statistically():
         assert _combined_sigma(a.mean_energy, a.mean_energy_error, b.mean_energy, b.mean_energy_error)
         assert _combined_sigma(a.flux_drift_velocity[2], a.flux_drift_velocity_error[2],
                                b.flux_drift_velocity[2], b.flux_drift_velocity_error[2])
+
+
+# Published LoKI-MC values for the Reid ramp model (benchmark calculations of
+# LoKI-MC v1.0.0, table 1: 1e5 electrons, stopping at 0.1 % / 0.4 % / 0.8 %
+# relative standard deviations): mean energy (eV), bulk drift velocity (m/s),
+# reduced bulk transverse and longitudinal diffusion (1/(m s)), each with its
+# relative standard deviation in %.
+REID_PUBLISHED = {
+    1.0: ((0.1014, 0.06), (1.264e4, 0.43), (0.9637e24, 0.70), (0.7542e24, 0.68)),
+    12.0: ((0.2691, 0.10), (6.838e4, 0.09), (1.137e24, 0.44), (0.5679e24, 0.35)),
+    24.0: ((0.4081, 0.10), (8.881e4, 0.07), (1.138e24, 0.53), (0.4595e24, 0.56)),
+}
+
+
+def _published_digit(v):
+    """Half a unit of the last printed digit of a 4-significant-digit value."""
+    return 0.5 * 10 ** (np.floor(np.log10(abs(v))) - 3)
+
+
+def test_reid_ramp_matches_published_loki_mc_values():
+    """Whole loop on the reference's own Reid-ramp input (its LXCat file and
+    setup: 4 amu, 133.32 Pa, 300 K, E/N = 1, 12, 24 Td batched in one FP64
+    engine): mean energy, bulk drift velocity and reduced bulk diffusion
+    within three combined standard deviations (plus the rounding of the
+    published four digits) of the published LoKI-MC results."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "reid_ramp_LXCat.txt")
+    ents = ekmc.read_lxcat(path)
+    procs = []
+    for ent in ents:
+        procs.append(dict(type=0, elastic=ent["type"] == "Elastic", threshold=ent["threshold"],
+                          mass_amu=4.0, energy=ent["energy"], sigma=ent["sigma"]))
+    mix = mixture(procs)
+    conds = [condition(en=en, n_e=100000) for en in sorted(REID_PUBLISHED)]
+    num = ekmc.numerics_default()
+    num.n_integration_points = 1e4
+    num.n_integrated_ss_times = 2
+    num.seed = 23
+    eng = _engine(mix, conds, num)
+    eng.solve()
+    for c, en in enumerate(sorted(REID_PUBLISHED)):
+        r = eng.results(c)
+        pub = REID_PUBLISHED[en]
+        dzz = r.bulk_diffusion_error[8] / abs(r.bulk_diffusion[8])
+        dxy = max(r.bulk_diffusion_error[0] / abs(r.bulk_diffusion[0]),
+                  r.bulk_diffusion_error[4] / abs(r.bulk_diffusion[4]))
+        ours = ((r.mean_energy, r.mean_energy_error / r.mean_energy),
+                (abs(r.bulk_drift_velocity[2]), r.bulk_drift_velocity_error[2] / abs(r.bulk_drift_velocity[2])),
+                (r.bulk_red_transv_diff, dxy),
+                (r.bulk_red_long_diff, dzz))
+        for name, (v, rel), (pv, prel) in zip(("mean energy", "bulk drift", "N D_T", "N D_L"), ours, pub):
+            tol = 3.0 * np.hypot(rel * v, prel / 100.0 * pv) + _published_digit(pv)
+            assert abs(v - pv) <= tol, (en, name, v, pv, tol)
+    eng.close()
