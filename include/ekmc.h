@@ -260,6 +260,11 @@ ekmc_status ekmc_get_series(const ekmc_engine* engine, int32_t cond, int32_t* n_
                             double* times, double* mean_energy, double* mean_position,
                             double* mean_velocity, double* flux_diffusion,
                             double* bulk_velocity, double* bulk_diffusion);
+/* Position covariance per sampling point, cov[9n] row-major
+ * (positionCovariances, BoltzmannMC.C:1437-1454; written to MCTemporalInfo.txt
+ * by Output.h:730-752).  *n_points as for ekmc_get_series. */
+ekmc_status ekmc_get_position_covariance(const ekmc_engine* engine, int32_t cond, int32_t* n_points,
+                                         double* cov);
 /* Copy the current electron ensemble of a condition to host (SoA, FP64). */
 ekmc_status ekmc_get_electrons(const ekmc_engine* engine, int32_t cond, double* pos /*3N*/,
                                double* vel /*3N*/, double* energy /*N*/, double* time /*N*/);

@@ -1,5 +1,7 @@
 """Build step of the engine: compiles the gfx950 kernels and the C++ host
-engine into loki-mc_amd/lib/libekmc.so with hipcc.
+engine into loki-mc_amd/lib/libekmc.so with hipcc, and the command-line
+driver (setup and LXCat readers, output writers) into
+loki-mc_amd/bin/lokimc_gpu with g++ against that library.
 
     python loki-mc_amd/build.py
 
@@ -14,6 +16,10 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "lib")
+BIN_DIR = os.path.join(HERE, "bin")
+HOST_DIR = os.path.join(HERE, "host")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+HOST_SOURCES = ["setup_file.cpp", "gas_model.cpp", "outputs.cpp", "lokimc_gpu.cpp"]
 OBJ_DIR = os.path.join(LIB_DIR, "obj")
 ARCH = os.environ.get("EKMC_OFFLOAD_ARCH", "gfx950")
 
@@ -48,7 +54,22 @@ def build(verbose=False):
     tmp = out + ".tmp"
     _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs, verbose)
     os.replace(tmp, out)
+    build_cli(verbose)
     return out
+
+
+def build_cli(verbose=False):
+    """The command-line driver: host C++ only, linked against libekmc.so."""
+    os.makedirs(BIN_DIR, exist_ok=True)
+    exe = os.path.join(BIN_DIR, "lokimc_gpu")
+    tmp = exe + ".tmp"
+    cmd = (["g++", "-std=c++17", "-O2", "-Wall", "-I", INCLUDE] +
+           [os.path.join(HOST_DIR, s) for s in HOST_SOURCES] +
+           ["-L", LIB_DIR, "-lekmc", "-Wl,-rpath,$ORIGIN/../lib", "-L/opt/rocm/lib",
+            "-Wl,-rpath-link,/opt/rocm/lib", "-o", tmp])
+    _run(cmd, verbose)
+    os.replace(tmp, exe)
+    return exe
 
 
 def _run(cmd, verbose):

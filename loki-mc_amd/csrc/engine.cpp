@@ -1482,6 +1482,16 @@ ekmc_status ekmc_get_series(const ekmc_engine* e, int32_t cond, int32_t* n_point
     return EKMC_OK;
 }
 
+ekmc_status ekmc_get_position_covariance(const ekmc_engine* e, int32_t cond, int32_t* n_points, double* cov) {
+    if (!e || !n_points || cond < 0 || cond >= (int)e->impl->cs.size()) return fail(EKMC_ERR_INVALID_ARGUMENT, "bad argument");
+    const Cond& C = e->impl->cs[cond];
+    const int n = std::min(*n_points, C.nSamp);
+    if (cov)
+        for (int i = 0; i < 9 * n; ++i) cov[i] = C.sCov[i];
+    *n_points = C.nSamp;
+    return EKMC_OK;
+}
+
 ekmc_status ekmc_get_electrons(const ekmc_engine* e, int32_t cond, double* pos, double* vel, double* energy,
                                double* time) {
     if (!e || cond < 0 || cond >= (int)e->impl->cs.size()) return fail(EKMC_ERR_INVALID_ARGUMENT, "bad argument");

@@ -1,0 +1,164 @@
+"""The command-line driver (setup-file and LXCat readers, output writers).
+
+CPU: --dry-run on the reference's Reid-ramp setup file and LXCat file (kept
+as data fixtures) and on small synthetic setups, checked against values fixed
+by the setup text and by the reference's parsing rules (Parse.C:18-295,
+590-742; FieldInfo.C:12-105; WorkingConditions.h:56-142).
+GPU: a full run of the Reid-ramp setup; its swarmParameters.txt against the
+published LoKI-MC values.
+"""
+import json
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "loki-mc_amd", "bin", "lokimc_gpu")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(EXE), reason="lokimc_gpu not built")
+
+
+def _input_tree(tmp_path):
+    inp = tmp_path / "Input"
+    (inp / "ReidRampGas").mkdir(parents=True)
+    shutil.copy(os.path.join(GOLDEN, "reid_ramp_LXCat.txt"), inp / "ReidRampGas" / "ReidRampGas_LXCat.txt")
+    shutil.copy(os.path.join(GOLDEN, "reid_ramp_setup_DC.in"), inp / "ReidRamp_setup_DC.in")
+    return inp
+
+
+def _run(args, cwd):
+    return subprocess.run([EXE] + args, cwd=cwd, capture_output=True, text=True, timeout=300)
+
+
+def _dry(setup_text, tmp_path):
+    inp = _input_tree(tmp_path)
+    (inp / "t.in").write_text(setup_text)
+    r = _run([str(inp / "t.in"), "--input", str(inp), "--dry-run"], tmp_path)
+    return r
+
+
+REID_TEXT = open(os.path.join(GOLDEN, "reid_ramp_setup_DC.in")).read()
+
+
+def test_reid_ramp_setup_parses_like_the_reference(tmp_path):
+    inp = _input_tree(tmp_path)
+    r = _run([str(inp / "ReidRamp_setup_DC.in"), "--input", str(inp), "--dry-run"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    assert d["variable"] == "reducedElecField"
+    assert [j["reduced_elec_field"] for j in d["jobs"]] == [1.0, 12.0, 24.0]
+    N = 133.32 / (1.38064852e-23 * 300.0)       # gasDensity = p / (k T), WorkingConditions.h:60
+    for j in d["jobs"]:
+        assert j["gas_density"] == N and j["n_electrons"] == 100000
+        assert j["elec_field_angle"] == 180.0 and j["reduced_mag_field"] == 0.0
+    # mass "4.000000*1.660539040e-27" evaluated and rounded to 16 digits
+    assert d["gases"] == [{"name": "A", "mass": float("%.16g" % (4.0 * 1.660539040e-27)), "fraction": 1.0}]
+    assert d["numerics"]["n_integration_points"] == 1e4 and d["numerics"]["n_integrated_ss_times"] == 2
+    assert d["numerics"]["gas_temperature_effect"] == 0 and d["numerics"]["ionization_operator"] == 0
+    p = d["processes"]
+    assert [x["description"] for x in p] == ["e+A(gnd)->e+A(gnd),Elastic", "e+A(gnd)->e+A(exc),Excitation"]
+    assert [x["elastic"] for x in p] == [1, 0] and [x["threshold"] for x in p] == [0.0, 0.2]
+    assert [x["rel_density"] for x in p] == [1.0, 1.0]
+
+
+@pytest.mark.parametrize("expr,values", [
+    ("[1,12,24]", [1, 12, 24]),
+    ("linspace(0,10,6)", [0, 2, 4, 6, 8, 10]),
+    ("logspace(-1,1,3)", [0.1, 1, 10]),
+    ("2:5", [2, 3, 4, 5]),
+    ("1:0.5:2", [1, 1.5, 2]),
+    ("[2^3^2,-2^2,sqrt(16)*2,3!,1e1/4]", [64, 4, 8, 6, 2.5]),
+])
+def test_reduced_field_expressions(tmp_path, expr, values):
+    """Vector expressions (Parse.C:590-652) and the expression precedence of
+    the reference's parser: '^' left to right, unary minus binding tighter
+    than '^', postfix factorial."""
+    r = _dry(REID_TEXT.replace("[1,12,24]", expr), tmp_path)
+    assert r.returncode == 0, r.stderr
+    got = [j["reduced_elec_field"] for j in json.loads(r.stdout)["jobs"]]
+    assert np.allclose(got, values, rtol=1e-15, atol=0)
+
+
+def test_comments_indentation_and_property_files(tmp_path):
+    inp = _input_tree(tmp_path)
+    (inp / "masses.txt").write_text("% name mass\nA 6.6e-27   % comment\nB 1\n")
+    text = REID_TEXT.replace("    mass:\n      - A = 4.000000*1.660539040e-27", "    mass: masses.txt")
+    (inp / "t.in").write_text(text)
+    r = _run([str(inp / "t.in"), "--input", str(inp), "--dry-run"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert json.loads(r.stdout)["gases"][0]["mass"] == 6.6e-27
+
+
+def test_reversible_collision_gets_a_superelastic_partner(tmp_path):
+    inp = _input_tree(tmp_path)
+    (inp / "g.txt").write_text(
+        "PROCESS: E + X(gnd) -> E + X(gnd), Elastic\nPARAM.: m/M = 1e-4\nCOMMENT: [e + X(gnd) -> e + X(gnd), Elastic]\n"
+        "-----\n0 1e-20\n100 1e-20\n-----\n"
+        "PROCESS: E + X(gnd) <-> E + X(exc), Excitation\nPARAM.: E = 1.5 eV\nCOMMENT: [e + X(gnd) <-> e + X(exc), Excitation]\n"
+        "-----\n1.5 0\n100 1e-20\n-----\n")
+    text = (REID_TEXT.replace("ReidRampGas/ReidRampGas_LXCat.txt", "g.txt").replace("- A = 4.000000*1.660539040e-27", "- X = 1e-26")
+            .replace("- A = 1", "- X = 1").replace("- A(gnd) = 1.0", "- X(gnd) = 0.9\n      - X(exc) = 0.1\n    statisticalWeight:\n      - X(gnd) = 1\n      - X(exc) = 3"))
+    (inp / "t.in").write_text(text)
+    r = _run([str(inp / "t.in"), "--input", str(inp), "--dry-run"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    p = json.loads(r.stdout)["processes"]
+    assert [x["superelastic"] for x in p] == [0, 0, 1]
+    assert p[2]["description"] == p[1]["description"] == "e+X(gnd)<->e+X(exc),Excitation"
+    assert p[1]["rel_density"] == 0.9 and p[2]["rel_density"] == 0.1     # target / product densities
+    assert p[2]["stat_weight_ratio"] == 1.0 / 3.0 and p[2]["points"] == 0
+
+
+@pytest.mark.parametrize("change,message", [
+    (("Elastic", "Elastik"), "Invalid type of collision"),
+    (("eedfType: boltzmannMC", "eedfType: boltzmann"), "boltzmannMC"),
+    (("- A = 4.000000*1.660539040e-27", "- B = 1"), "Mass of gas A not found"),
+    (("reducedMagField: 0", "reducedMagField: [0,1]"), "multiple values"),
+])
+def test_input_errors_end_the_run_with_the_message(tmp_path, change, message):
+    inp = _input_tree(tmp_path)
+    lx = (inp / "ReidRampGas" / "ReidRampGas_LXCat.txt")
+    if change[0] == "Elastic":
+        lx.write_text(lx.read_text().replace("A(gnd), Elastic]", "A(gnd), Elastik]"))
+        text = REID_TEXT
+    else:
+        text = REID_TEXT.replace(*change)
+    (inp / "t.in").write_text(text)
+    r = _run([str(inp / "t.in"), "--input", str(inp), "--dry-run"], tmp_path)
+    assert r.returncode == 1 and message in r.stderr
+    assert message in (tmp_path / "errorLog.txt").read_text()
+
+
+# Published LoKI-MC values (benchmark calculations of LoKI-MC v1.0.0, table 1).
+REID = {1: (0.1014, 0.06), 12: (0.2691, 0.10), 24: (0.4081, 0.10)}
+
+
+@pytest.mark.gpu
+def test_reid_ramp_run_writes_the_reference_output_files(tmp_path):
+    inp = _input_tree(tmp_path)
+    r = _run([str(inp / "ReidRamp_setup_DC.in"), "--input", str(inp), "--output", str(tmp_path / "Output"), "--seed", "5"],
+             tmp_path)
+    assert r.returncode == 0, r.stderr
+    out = tmp_path / "Output" / "ReidRamp_DC"
+    assert (out / "setup.txt").exists()
+    for en, (pub, rel) in REID.items():
+        sub = out / ("reducedElecField_%g" % en)
+        for name in ("eedf.txt", "swarmParameters.txt", "rateCoefficients.txt", "rateCoefficientsMC.txt",
+                     "powerBalance.txt", "MCSimDetails.txt"):
+            assert (sub / name).exists(), name
+        sw = (sub / "swarmParameters.txt").read_text()
+        m = re.search(r"Mean energy = (\S+) \(eV\) ; Rel. std: (\S+)%", sw)
+        mean, err = float(m.group(1)), float(m.group(2)) / 100.0
+        assert abs(mean - pub) <= 3.0 * np.hypot(err * mean, rel / 100.0 * pub) + 5e-5
+        eedf = np.loadtxt(sub / "eedf.txt", skiprows=1)
+        de = eedf[1, 0] - eedf[0, 0]
+        # normalisation of the EEDF: sum f(e) sqrt(e) de = 1
+        assert abs(np.sum(eedf[:, 1] * np.sqrt(eedf[:, 0])) * de - 1.0) < 1e-6
+        pb = (sub / "powerBalance.txt").read_text()
+        rel_balance = float(re.search(r"Relative Power Balance = (\S+)%", pb).group(1))
+        assert rel_balance < 1.0
+    assert (out / "lookUpTableSwarm.txt").exists()
