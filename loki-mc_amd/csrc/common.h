@@ -19,6 +19,11 @@ constexpr int kEnvBins = 256;          // bins of the null-test envelope of the 
 
 // Field cases of accelerateElectron (BoltzmannMC.C:809-897)
 enum FieldCase : int { FIELD_DC = 0, FIELD_AC = 1, FIELD_DC_B = 2, FIELD_ECR = 3, FIELD_AC_B = 4 };
+// Flight-kernel class of a launch that takes both DC and DC+B conditions
+// (the flight formula is chosen per chunk); a batch holding both shares one
+// work queue, so the end of the launch idles once instead of twice.
+constexpr int FLIGHT_DC_ANY = 5;
+constexpr int kFlightClasses = 6;
 
 // Philox counter purposes
 enum Purpose : uint32_t { P_EVENT = 0, P_INIT = 1, P_POPCTL = 2, P_CHILD = 3, P_CLONE = 4 };

@@ -682,7 +682,12 @@ __device__ __forceinline__ int step_a(const CondParams& c, const DevProcesses& p
     // one flight for both outcomes (a wave holds both kinds of lanes)
     const bool partial = e.t + (double)e.tfree > c.tsync;
     const double dt = partial ? c.tsync - e.t : (double)e.tfree;
-    flight<FC>(c, e, dt, gain);
+    if (FC == FLIGHT_DC_ANY) {
+        if (c.fieldCase == FIELD_DC) flight<FIELD_DC>(c, e, dt, gain);
+        else flight<FIELD_DC_B>(c, e, dt, gain);
+    } else {
+        flight<FC>(c, e, dt, gain);
+    }
     if (partial) {
         e.t = c.tsync;
         e.tfree = (V)((double)e.tfree - dt);

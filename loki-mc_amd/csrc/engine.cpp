@@ -237,6 +237,7 @@ public:
     std::vector<double> upGrid;
     std::vector<int32_t> upRow;
     bool fusedHist = true;            // running conditions' histograms counted by the flight kernel
+    bool mergeDC = true;              // DC and DC+B conditions in one flight launch
     DevBuf<ProcRec> prec;
     DevBuf<int> iproc;
     DevBuf<double> dproc;
@@ -304,6 +305,8 @@ public:
         // EKMC_FUSED_HIST=0: every histogram by the histogram kernel
         const char* fh = std::getenv("EKMC_FUSED_HIST");
         fusedHist = !(fh && fh[0] == '0');
+        const char* md = std::getenv("EKMC_MERGE_DC");
+        mergeDC = !(md && md[0] == '0');
         // HIP-event timing of the flight kernels; EKMC_KERNEL_TIMING=all times
         // every kernel, EKMC_KERNEL_TIMING=0 none
         const char* kt = std::getenv("EKMC_KERNEL_TIMING");
@@ -672,10 +675,20 @@ public:
         hlist.clear();
         fl = FlightLists{};
         const bool thermal = num.gas_temperature_effect != EKMC_GASTEMP_FALSE;
-        for (int fc = 0; fc < 5; ++fc) {
+        // DC and DC+B conditions share one launch when both are running
+        bool anyDC = false, anyDCB = false;
+        for (int c = 0; c < nCond; ++c) {
+            anyDC |= act[c] && cs[c].fieldCase == FIELD_DC;
+            anyDCB |= act[c] && cs[c].fieldCase == FIELD_DC_B;
+        }
+        const bool merged = anyDC && anyDCB && mergeDC;
+        for (int fc = 0; fc < kFlightClasses; ++fc) {
             const size_t begin = hlist.size();
             for (int c = 0; c < nCond; ++c) {
-                if (!act[c] || cs[c].fieldCase != fc) continue;
+                if (!act[c]) continue;
+                const int cls = merged && (cs[c].fieldCase == FIELD_DC || cs[c].fieldCase == FIELD_DC_B) ? FLIGHT_DC_ANY
+                                                                                                      : cs[c].fieldCase;
+                if (cls != fc) continue;
                 for (int j = 0; j < hcp[c].nChunks; ++j) hlist.push_back(hcp[c].chunkBegin + j);
             }
             const int32_t count = (int32_t)(hlist.size() - begin);

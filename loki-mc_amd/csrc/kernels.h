@@ -102,11 +102,12 @@ struct DevHist {
     int histLds;                  // EEDF cells the flight kernel counts in LDS (0: in HBM)
 };
 
-// Flight-kernel work lists, one per field case: chunk ids in processing order.
+// Flight-kernel work lists, one per field case (or FLIGHT_DC_ANY): chunk ids
+// in processing order.
 struct FlightLists {
-    const int32_t* list[5];
-    int32_t count[5];
-    int32_t grid[5];              // persistent blocks per launch
+    const int32_t* list[kFlightClasses];
+    int32_t count[kFlightClasses];
+    int32_t grid[kFlightClasses];  // persistent blocks per launch
 };
 
 // Sources of the collision tables (interpolateCrossSections,
@@ -150,7 +151,7 @@ hipError_t launch_sample(const DevState<V>& s, const DevWork& w, hipStream_t st)
 template <typename V>
 hipError_t launch_hist(const DevState<V>& s, const DevHist& h, hipStream_t st);
 hipError_t launch_finish(const DevWork& w, int P, hipStream_t st);
-// Resident blocks per CU of the flight kernel of field case fc.
+// Resident blocks per CU of the flight kernel of field case (or class) fc.
 template <typename V>
 int flight_blocks_per_cu(int fc, bool thermal, int P, int histLds);
 // Static LDS of one flight-kernel block: the record ring, two halves of 32

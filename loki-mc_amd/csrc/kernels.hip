@@ -1381,6 +1381,9 @@ int flight_blocks_per_cu(int fc, bool thermal, int P, int histLds) {
         case FIELD_AC: k = thermal ? flight_kernel<V, FIELD_AC, true>() : flight_kernel<V, FIELD_AC, false>(); break;
         case FIELD_DC_B: k = thermal ? flight_kernel<V, FIELD_DC_B, true>() : flight_kernel<V, FIELD_DC_B, false>(); break;
         case FIELD_ECR: k = thermal ? flight_kernel<V, FIELD_ECR, true>() : flight_kernel<V, FIELD_ECR, false>(); break;
+        case FLIGHT_DC_ANY:
+            k = thermal ? flight_kernel<V, FLIGHT_DC_ANY, true>() : flight_kernel<V, FLIGHT_DC_ANY, false>();
+            break;
         default: k = thermal ? flight_kernel<V, FIELD_AC_B, true>() : flight_kernel<V, FIELD_AC_B, false>(); break;
     }
     int n = 0;
@@ -1398,6 +1401,7 @@ hipError_t launch_interval(const DevState<V>& s, const DevWork& w, const DevProc
     launch_advance_case<V, FIELD_DC_B>(s, w, pr, tables, h, fl, st);
     launch_advance_case<V, FIELD_ECR>(s, w, pr, tables, h, fl, st);
     launch_advance_case<V, FIELD_AC_B>(s, w, pr, tables, h, fl, st);
+    launch_advance_case<V, FLIGHT_DC_ANY>(s, w, pr, tables, h, fl, st);
     timer_stop(K_ADVANCE, st);
     timer_start(K_PLAN, st);
     hipLaunchKernelGGL(k_plan, dim3(w.nCond), dim3(1024), 0, st, w);
