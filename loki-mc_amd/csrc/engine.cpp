@@ -691,6 +691,11 @@ public:
                 if (cls != fc) continue;
                 for (int j = 0; j < hcp[c].nChunks; ++j) hlist.push_back(hcp[c].chunkBegin + j);
             }
+            // larger chunks first, so that the last chunks taken are the
+            // short ones and the waves finish together (the processing order
+            // does not change any result)
+            std::stable_sort(hlist.begin() + begin, hlist.end(),
+                             [&](int32_t a, int32_t b) { return hchunks[a].count > hchunks[b].count; });
             const int32_t count = (int32_t)(hlist.size() - begin);
             fl.list[fc] = flist.p + begin;
             fl.count[fc] = count;
