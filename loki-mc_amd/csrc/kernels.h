@@ -154,9 +154,15 @@ hipError_t launch_finish(const DevWork& w, int P, hipStream_t st);
 // Resident blocks per CU of the flight kernel of field case (or class) fc.
 template <typename V>
 int flight_blocks_per_cu(int fc, bool thermal, int P, int histLds);
-// Static LDS of one flight-kernel block: the record ring, two halves of 32
-// records per wave.
-template <typename V> constexpr size_t flight_ring_lds() { return (size_t)2 * (kBlock / kWave) * 32 * sizeof(Rec<V>); }
+// Record ring of the flight kernel: segments of kRingSeg main records per
+// wave, ring_segs<V>() of them (the mixed record is smaller, so its ring is
+// deeper in the same LDS).
+constexpr int kRingSeg = 32;
+template <typename V> constexpr int ring_segs() { return sizeof(V) == 4 ? 3 : 2; }
+// Static LDS of one flight-kernel block: the record rings of its waves.
+template <typename V> constexpr size_t flight_ring_lds() {
+    return (size_t)ring_segs<V>() * (kBlock / kWave) * kRingSeg * sizeof(Rec<V>);
+}
 
 template <typename V>
 hipError_t launch_op_accelerate(const CondParams* cond, int64_t n, const double* time, const double* dt,

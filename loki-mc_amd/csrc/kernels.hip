@@ -126,37 +126,59 @@ template <typename V> __device__ __forceinline__ double rec_energy(const Rec<V>&
 }
 
 // ------------------------------------------------------------ record ring
-// The main records of a chunk stream through a ring of 2 x kRingHalf records
-// per wave in LDS, filled by direct-to-LDS loads issued at least half a ring
-// ahead of use: a lane taking the next main electron reads LDS, and the HBM
-// latency of the record overlaps the flights of the wave.  Chunk-relative
-// main record i lives in half (i / kRingHalf) % 2, slot i % kRingHalf; piece k
-// (16 bytes) of slot j is half[k * kRingHalf + j].  Lane j < kRingHalf issues
-// the loads of slot j (the LDS address a lane writes is the base plus
-// 16 * lane).  The wave orders the ring itself: the first read of a half
-// after its fill waits for every outstanding vector memory operation
-// (vmcnt(0)), and a read completes inside its own instruction group, so a
-// later fill of that half cannot overtake it.
-constexpr int kRingHalf = kWave / 2;
-static_assert(flight_ring_lds<float>() == 2 * kWavesPerBlock * kRingHalf * sizeof(Rec<float>), "ring size");
+// The main records of a chunk stream through a per-wave ring in LDS of
+// ring_segs<V>() segments of kRingSeg records, filled by direct-to-LDS loads
+// issued up to ring_segs - 1 segments ahead of use: a lane taking the next
+// main electron reads LDS, and the HBM latency of the record overlaps the
+// flights of the wave.  Fill f of a chunk loads chunk-relative main records
+// [f kRingSeg, (f + 1) kRingSeg) into segment f % ring_segs; piece k
+// (16 bytes) of slot j of a segment is seg[k * kRingSeg + j].  Lane
+// j < kRingSeg issues the loads of slot j (the LDS address a lane writes is
+// the base plus 16 * lane), so every fill is rec_pieces vector memory
+// instructions (lane 0 always has a record).  A wave's vector memory
+// operations complete in issue order (vmcnt), so fill f has landed once at
+// most rec_pieces * (fills issued after f) operations are outstanding: the
+// first read of a segment waits for exactly that, not for the younger fills.
+// A read completes inside its own instruction group, so a later fill of that
+// segment cannot overtake it.
+static_assert(kRingSeg == kWave / 2, "one ring slot per lane of the lower half");
 template <typename V> __host__ __device__ constexpr int rec_pieces() { return (int)(sizeof(Rec<V>) / 16); }
+static_assert(flight_ring_lds<float>() == (size_t)ring_segs<float>() * kWavesPerBlock * kRingSeg * sizeof(Rec<float>),
+              "ring size");
 typedef __attribute__((address_space(1))) void global_void;
 typedef __attribute__((address_space(3))) void lds_void;
 typedef __attribute__((address_space(3))) uint4 lds_uint4;  // LDS pointers stay LDS pointers
 
-// Loads of main records [loaded, loaded + kRingHalf) of a chunk of `count`
-// into `half`.
+template <typename V> __device__ __forceinline__ lds_uint4* ring_segment(lds_uint4* ring, int record) {
+    return ring + ((record / kRingSeg) % ring_segs<V>()) * (rec_pieces<V>() * kRingSeg);
+}
+// Loads of main records [loaded, loaded + kRingSeg) of a chunk of `count`
+// into their segment.
 template <typename V>
-__device__ __forceinline__ void ring_fill(const Rec<V>* rec, int64_t first, int count, int loaded, lds_uint4* half) {
+__device__ __forceinline__ void ring_fill(const Rec<V>* rec, int64_t first, int count, int loaded, lds_uint4* ring) {
     const int lane = lane_id();
     const int idx = loaded + lane;
-    if (lane < kRingHalf && idx < count) {
+    lds_uint4* seg = ring_segment<V>(ring, loaded);
+    if (lane < kRingSeg && idx < count) {
         const char* src = (const char*)(rec + first + idx);
 #pragma unroll
         for (int k = 0; k < rec_pieces<V>(); ++k)
-            __builtin_amdgcn_global_load_lds((global_void*)(src + 16 * k), (lds_void*)(half + k * kRingHalf), 16, 0,
-                                             0);
+            __builtin_amdgcn_global_load_lds((global_void*)(src + 16 * k), (lds_void*)(seg + k * kRingSeg), 16, 0, 0);
     }
+}
+// s_waitcnt with vmcnt(N) (expcnt and lgkmcnt not waited for)
+template <int N> __device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+// Wait until a fill followed by `later` younger fills has landed.
+template <typename V> __device__ __forceinline__ void ring_wait(int later) {
+    constexpr int p = rec_pieces<V>();
+    static_assert(ring_segs<V>() <= 4, "ring_wait covers at most 3 younger fills");
+    if (later <= 0) wait_vmcnt<0>();
+    else if (later == 1) wait_vmcnt<p>();
+    else if (later == 2) wait_vmcnt<2 * p>();
+    else wait_vmcnt<3 * p>();
 }
 // The pieces of one ring slot at LDS byte address `a`, read and waited for
 // in one group.
@@ -184,17 +206,17 @@ __device__ __forceinline__ void ring_pieces(uint32_t a, uint4 (&p)[6]) {
         : "v"(a)
         : "memory");
 }
-static_assert(kRingHalf * 16 == 512, "ring piece stride");
+static_assert(kRingSeg * 16 == 512, "ring piece stride");
 // LDS atomic add without return, issued as is: the compiler would otherwise
 // order it after the outstanding ring loads (it cannot tell the histogram
 // from the ring), which would expose their HBM latency.
 __device__ __forceinline__ void lds_add_u32(uint32_t* p, uint32_t v) {
     asm volatile("ds_add_u32 %0, %1" : : "v"((uint32_t)(size_t)(lds_void*)p), "v"(v) : "memory");
 }
-// Slot j of a ring half.
-template <typename V> __device__ __forceinline__ void ring_read(const lds_uint4* half, int j, Electron<V>& e) {
+// Slot j of a ring segment.
+template <typename V> __device__ __forceinline__ void ring_read(const lds_uint4* seg, int j, Electron<V>& e) {
     uint4 p[rec_pieces<V>()];
-    ring_pieces((uint32_t)(size_t)(half + j), p);
+    ring_pieces((uint32_t)(size_t)(seg + j), p);
     Rec<V> q;
     __builtin_memcpy(&q, p, sizeof(q));
     e.x = q.x; e.y = q.y; e.z = q.z; e.t = q.t;
@@ -393,7 +415,7 @@ __device__ __forceinline__ void hist_count(const DevHist& h, int c, const double
 template <typename V, int FC, bool THERMAL>
 __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWork& w, const DevProcesses& pr,
                                               const V* tables, const DevHist& h, int cid, unsigned long long* cnt,
-                                              lds_uint4* ringA, lds_uint4* ringB) {
+                                              lds_uint4* ring) {
     const int lane = lane_id();
     const int P = pr.P;
     long long* gainL = (long long*)(cnt + P);
@@ -426,11 +448,10 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
     const int count = ch.count;
     int nextMain = 0, nq = 0, nextNew = 0, holes = 0;
     int loaded = 0;  // main records whose ring loads are issued
-    bool pendA = false, pendB = false;  // a half filled since the last vmcnt(0) of the wave
-    for (; loaded < count && loaded <= nextMain + kRingHalf; loaded += kRingHalf) {
-        if ((loaded / kRingHalf) & 1) { ring_fill<V>(s.rec, ch.first, count, loaded, ringB); pendB = true; }
-        else { ring_fill<V>(s.rec, ch.first, count, loaded, ringA); pendA = true; }
-    }
+    int landed = 0;  // ring fills known to have landed
+    constexpr int kAhead = (ring_segs<V>() - 1) * kRingSeg;
+    for (; loaded < count && loaded <= nextMain + kAhead; loaded += kRingSeg)
+        ring_fill<V>(s.rec, ch.first, count, loaded, ring);
     Electron<V> e;
     bool have = false, isNew = false, parked = false;
     int64_t slot = 0;
@@ -448,13 +469,12 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
             const int availNew = nq - nextNew;
             const int takeNew = (nNeed - takeMain) < availNew ? (nNeed - takeMain) : availNew;
             if (takeMain > 0) {
-                // main records [nextMain, nextMain + takeMain) span at most two halves
-                const int h0 = (nextMain / kRingHalf) & 1, h1 = ((nextMain + takeMain - 1) / kRingHalf) & 1;
-                const bool useA = h0 == 0 || h1 == 0 || takeMain > kRingHalf;
-                const bool useB = h0 == 1 || h1 == 1 || takeMain > kRingHalf;
-                if ((useA && pendA) || (useB && pendB)) {
-                    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): the ring loads have landed
-                    pendA = pendB = false;
+                // main records [nextMain, nextMain + takeMain): the youngest
+                // fill they need, and the fills issued after it
+                const int need = (nextMain + takeMain - 1) / kRingSeg;
+                if (need >= landed) {
+                    ring_wait<V>(loaded / kRingSeg - 1 - need);
+                    landed = need + 1;
                 }
             }
             if (!have) {
@@ -473,8 +493,7 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
                         load_rec(s.rec, slot, e);
                     } else {
                         const int idx = (int)(slot - ch.first);
-                        if ((idx / kRingHalf) & 1) ring_read<V>(ringB, idx & (kRingHalf - 1), e);
-                        else ring_read<V>(ringA, idx & (kRingHalf - 1), e);
+                        ring_read<V>(ring_segment<V>(ring, idx), idx % kRingSeg, e);
                     }
                     iter = 0;
                     if (histOn && !isNew) hist_count<V>(h, c, hg, e, histLds ? lhist : nullptr, histRow, histCyl);
@@ -574,12 +593,9 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
             have = false;
         }
         holes += __popcll(dm);
-        // refill a consumed ring half, after this step's memory waits (a later
-        // wait on a vector load covers the loads issued before it)
-        for (; loaded < count && loaded <= nextMain + kRingHalf; loaded += kRingHalf) {
-            if ((loaded / kRingHalf) & 1) { ring_fill<V>(s.rec, ch.first, count, loaded, ringB); pendB = true; }
-            else { ring_fill<V>(s.rec, ch.first, count, loaded, ringA); pendA = true; }
-        }
+        // refill consumed ring segments
+        for (; loaded < count && loaded <= nextMain + kAhead; loaded += kRingSeg)
+            ring_fill<V>(s.rec, ch.first, count, loaded, ring);
     }
 
     int remaining = 0;
@@ -633,18 +649,17 @@ __global__ __launch_bounds__(kBlock) void k_advance(DevState<V> s, DevWork w, De
                                                     DevHist h, const int32_t* list, int32_t nList,
                                                     uint32_t* queue) {
     extern __shared__ unsigned char smem[];
-    __shared__ uint4 ringsA[kWavesPerBlock * rec_pieces<V>() * kRingHalf];
-    __shared__ uint4 ringsB[kWavesPerBlock * rec_pieces<V>() * kRingHalf];
+    constexpr int kRingWave = ring_segs<V>() * rec_pieces<V>() * kRingSeg;  // 16-byte pieces per wave
+    __shared__ uint4 rings[kWavesPerBlock * kRingWave];
     const int wid = threadIdx.x / kWave;
     unsigned long long* cnt = (unsigned long long*)(smem + (size_t)wid * flight_wave_lds<V>(pr.P, h.histLds));
-    lds_uint4* ringA = (lds_uint4*)(ringsA + wid * rec_pieces<V>() * kRingHalf);
-    lds_uint4* ringB = (lds_uint4*)(ringsB + wid * rec_pieces<V>() * kRingHalf);
+    lds_uint4* ring = (lds_uint4*)(rings + wid * kRingWave);
     while (true) {
         uint32_t item = 0;
         if (lane_id() == 0) item = atomicAdd(queue, 1u);
         item = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl((int)item, 0, kWave));
         if (item >= (uint32_t)nList) break;
-        advance_chunk<V, FC, THERMAL>(s, w, pr, tables, h, list[item], cnt, ringA, ringB);
+        advance_chunk<V, FC, THERMAL>(s, w, pr, tables, h, list[item], cnt, ring);
     }
 }
 
