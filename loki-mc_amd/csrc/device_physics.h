@@ -402,12 +402,28 @@ __device__ inline int channel_search(const V* c1, const V* c2, const V* i1, cons
 #pragma clang fp contract(off)
     int chosen = -1;
     bool broke = false;
+    auto probe = [&](int t) -> V {
+        return perGas ? prevSum + (w1 * c1[t] + w2 * c2[t] - ref) * speed : w1 * c1[t] + w2 * c2[t];
+    };
+    // The probes of the sequential loop (t = (L + R) / 2, then the left or
+    // right half), two levels per round trip: the middles of both halves are
+    // loaded with the current middle, so a search makes half the dependent
+    // loads.  Every speculative index lies in [L, R].  (Three levels would
+    // raise the flight kernel above the registers of three waves per SIMD.)
     while (L != R) {
-        int t = (L + R) / 2;
-        V tv = perGas ? prevSum + (w1 * c1[t] + w2 * c2[t] - ref) * speed : w1 * c1[t] + w2 * c2[t];
-        if (target < tv) R = t;
-        else if (target > tv) L = t + 1;
-        else { chosen = t; broke = true; break; }
+        const int t0 = (L + R) / 2;
+        const int a1 = (L + t0) / 2, b1 = (t0 + 1 + R) / 2;
+        const V v0 = probe(t0), va1 = probe(a1), vb1 = probe(b1);
+        if (target < v0) R = t0;
+        else if (target > v0) L = t0 + 1;
+        else { chosen = t0; broke = true; break; }
+        if (L == R) break;
+        const bool left0 = R == t0;  // range [L, t0], middle a1; else [t0 + 1, R], middle b1
+        const int t1 = left0 ? a1 : b1;
+        const V v1 = left0 ? va1 : vb1;
+        if (target < v1) R = t1;
+        else if (target > v1) L = t1 + 1;
+        else { chosen = t1; broke = true; break; }
     }
     if (!broke) chosen = L;
     while (chosen > 0 && w1 * i1[chosen] == (V)0 && w2 * i2[chosen] == (V)0) --chosen;

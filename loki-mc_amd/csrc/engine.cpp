@@ -355,9 +355,12 @@ public:
             if (prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;
             if (prop.sharedMemPerBlock > 0) ldsMax = prop.sharedMemPerBlock;
         }
-        // per-wave LDS of the flight kernel (tallies, histogram grid, envelope)
-        // and its record ring; the EEDF is counted in LDS only where it fits
-        if ((size_t)(kBlock / kWave) * (3 * P * 8 + 64 + 2 * kEnvBins * sizeof(V)) + flight_ring_lds<V>() > ldsMax)
+        // LDS of a flight-kernel block: the process records, the per-wave
+        // tallies, histogram grid and envelope, and the record rings; the EEDF
+        // is counted in LDS only where it fits
+        if ((size_t)P * sizeof(ProcRec) + (size_t)(kBlock / kWave) * (3 * P * 8 + 64 + 2 * kEnvBins * sizeof(V)) +
+                flight_ring_lds<V>() >
+            ldsMax)
             return fail(EKMC_ERR_INVALID_ARGUMENT, "too many processes for the per-wave collision tallies in LDS");
         if (num.work_chunk < 0 || num.reserved != 0) return fail(EKMC_ERR_INVALID_ARGUMENT, "invalid work_chunk/reserved");
         int64_t chunkTarget = num.work_chunk > 0 ? num.work_chunk : kChunkDefault;

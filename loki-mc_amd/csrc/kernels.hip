@@ -652,14 +652,23 @@ __global__ __launch_bounds__(kBlock) void k_advance(DevState<V> s, DevWork w, De
     constexpr int kRingWave = ring_segs<V>() * rec_pieces<V>() * kRingSeg;  // 16-byte pieces per wave
     __shared__ uint4 rings[kWavesPerBlock * kRingWave];
     const int wid = threadIdx.x / kWave;
-    unsigned long long* cnt = (unsigned long long*)(smem + (size_t)wid * flight_wave_lds<V>(pr.P, h.histLds));
+    // the process records in LDS: a collision reads its process's constants
+    // at LDS latency, outside the in-order queue of the ring loads
+    uint4* procL = (uint4*)smem;
+    for (int i = threadIdx.x; i < pr.P * (int)(sizeof(ProcRec) / 16); i += kBlock)
+        procL[i] = ((const uint4*)pr.proc)[i];
+    __syncthreads();
+    DevProcesses prl = pr;
+    prl.proc = (const ProcRec*)smem;
+    unsigned long long* cnt =
+        (unsigned long long*)(smem + (size_t)pr.P * sizeof(ProcRec) + (size_t)wid * flight_wave_lds<V>(pr.P, h.histLds));
     lds_uint4* ring = (lds_uint4*)(rings + wid * kRingWave);
     while (true) {
         uint32_t item = 0;
         if (lane_id() == 0) item = atomicAdd(queue, 1u);
         item = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl((int)item, 0, kWave));
         if (item >= (uint32_t)nList) break;
-        advance_chunk<V, FC, THERMAL>(s, w, pr, tables, h, list[item], cnt, ring);
+        advance_chunk<V, FC, THERMAL>(s, w, prl, tables, h, list[item], cnt, ring);
     }
 }
 
@@ -1357,8 +1366,10 @@ __global__ __launch_bounds__(kBlock) void k_op_philox(int64_t n, const uint32_t*
 
 // ------------------------------------------------------------- launchers
 static inline unsigned chunk_grid(int nChunks) { return (unsigned)((nChunks + kWavesPerBlock - 1) / kWavesPerBlock); }
+// Dynamic LDS of a flight-kernel block: the process records [P], then the
+// per-wave areas.
 template <typename V> static inline size_t flight_lds(int P, int histLds) {
-    return (size_t)kWavesPerBlock * flight_wave_lds<V>(P, histLds);
+    return (size_t)P * sizeof(ProcRec) + (size_t)kWavesPerBlock * flight_wave_lds<V>(P, histLds);
 }
 
 template <typename V>
