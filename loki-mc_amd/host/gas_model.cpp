@@ -3,6 +3,7 @@
 
 #include <cctype>
 #include <cmath>
+#include <cstdio>
 #include <fstream>
 #include <map>
 
@@ -202,18 +203,52 @@ std::vector<LXCatEntry> readLXCat(const std::vector<std::string>& files, const s
 // ------------------------------------------------------------- mixture
 namespace {
 
+constexpr double kNonDef = -123456789.0;          // Constant::NON_DEF
+constexpr double kElectronCharge = 1.6021766208e-19;  // Constant.h:7
+constexpr double kBoltzmannJ = 1.38064852e-23;        // Constant.h:6
+constexpr double kPlanck = 6.626070040e-34;           // Constant.h:13
+constexpr double kPi = 3.14159265358979323846;
+const double kBoltzmannInEV = kBoltzmannJ / kElectronCharge;        // Constant.h:19
+const double kPlanckReducedInEV = kPlanck / (2.0 * kPi * kElectronCharge);  // Constant.h:20
+constexpr double kEps = 2.220446049250313e-16;
+
+// A state with its family (State.h:20-75, EedfState.C:8-95).
 struct State {
     RawState raw;
     int gas = -1;
-    double population = 0, statWeight = -1;
-    bool popSet = false;
+    std::string type;  // ele, vib, rot, ion
+    int parent = -1;
+    std::vector<int> siblings, children;
+    double energy = kNonDef, statWeight = kNonDef, population = 0, density = 0;
+    bool isTarget = false;
+    std::vector<int> colls;  // collisions with this state as target or reverse product
 };
 
+struct XSec {
+    std::vector<double> e, s;
+    bool defined() const { return !e.empty(); }
+};
+
+// Collision.C:18-86
 struct Collision {
-    LXCatEntry e;
-    int gas = -1, target = -1, product = -1;
+    std::string type;
+    int target = -1;
+    std::vector<int> products;
+    std::vector<double> stoi;
+    bool isReverse = false, extra = false;
+    double threshold = 0;
+    XSec integral, momTransf;
     std::string scattering = "isotropic";
     std::vector<double> params;
+};
+
+struct Gas {
+    std::string name;
+    double mass = kNonDef, harmonicFrequency = kNonDef, anharmonicFrequency = kNonDef, rotationalConstant = kNonDef;
+    double fraction = 0, opb = kNonDef;
+    std::vector<int> states, colls;       // colls: non-extra, in creation order
+    std::map<int, double> effPop;         // populations for an Elastic derived from an Effective (by state)
+    bool effPopDefined = false;
 };
 
 int scatteringCode(const std::string& n) {
@@ -228,137 +263,741 @@ int scatteringCode(const std::string& n) {
 
 int requiredParams(const std::string& n) { return n == "coulombScreen" ? 2 : 0; }
 
-double propertyNumber(const std::string& key, const std::string& value, const std::string& what) {
-    if (value.find('@') != std::string::npos)
-        throw InputError("The " + what + " of '" + key + "' is given by a property function ('" + value +
-                         "'); property functions are not supported by this reader.");
-    return evalNumber(value);
+void warn(const std::string& msg) { std::fprintf(stderr, "%s\n", msg.c_str()); }
+
+struct Model {
+    std::vector<Gas> gases;
+    std::vector<State> states;
+    std::vector<Collision> colls;
+
+    int gasId(const std::string& g) {
+        for (size_t i = 0; i < gases.size(); ++i)
+            if (gases[i].name == g) return (int)i;
+        Gas x;
+        x.name = g;
+        gases.push_back(x);
+        return (int)gases.size() - 1;
+    }
+    int findExact(const RawState& r) const {
+        for (size_t i = 0; i < states.size(); ++i)
+            if (states[i].raw == r) return (int)i;
+        return -1;
+    }
+    // State::add with EedfState::addFamily: a new state's relatives among the
+    // gas's existing states, scanned in creation order.
+    int addState(const RawState& r) {
+        const int found = findExact(r);
+        if (found >= 0) return found;
+        const int g = gasId(r.gas);
+        State s;
+        s.raw = r;
+        s.gas = g;
+        s.type = !r.ionCharg.empty() ? "ion" : (!r.rotLevel.empty() ? "rot" : (!r.vibLevel.empty() ? "vib" : "ele"));
+        const int id = (int)states.size();
+        for (int o : gases[g].states) {
+            State& t = states[o];
+            const bool sameEle = t.raw.eleLevel == r.eleLevel, sameVib = t.raw.vibLevel == r.vibLevel;
+            if (s.type == "rot") {
+                if (t.type == "vib" && sameEle && sameVib) { s.parent = o; t.children.push_back(id); }
+                else if (t.type == "rot" && sameEle && sameVib) { s.siblings.push_back(o); t.siblings.push_back(id); }
+            } else if (s.type == "vib") {
+                if (t.type == "ele" && sameEle) { s.parent = o; t.children.push_back(id); }
+                else if (t.type == "vib" && sameEle) { s.siblings.push_back(o); t.siblings.push_back(id); }
+                else if (t.type == "rot" && sameEle && sameVib) { s.children.push_back(o); t.parent = id; }
+            } else if (s.type == "ele") {
+                if (t.type == "ele") { s.siblings.push_back(o); t.siblings.push_back(id); }
+                else if (t.type == "vib" && sameEle) { s.children.push_back(o); t.parent = id; }
+            } else if (t.type == "ion") {
+                s.siblings.push_back(o);
+                t.siblings.push_back(id);
+            }
+        }
+        states.push_back(s);
+        gases[g].states.push_back(id);
+        return id;
+    }
+    // State::fixOrphanStates: parents of rotational, then vibrational states
+    void fixOrphans() {
+        for (size_t i = 0; i < states.size(); ++i)
+            if (states[i].type == "rot" && states[i].parent < 0) {
+                RawState p = states[i].raw;
+                p.rotLevel.clear();
+                addState(p);
+            }
+        for (size_t i = 0; i < states.size(); ++i)
+            if (states[i].type == "vib" && states[i].parent < 0) {
+                RawState p = states[i].raw;
+                p.vibLevel.clear();
+                p.rotLevel.clear();
+                addState(p);
+            }
+    }
+    // State::findPointer: '*' as the electronic level takes every electronic
+    // state of the gas, as the vibrational level the children of the
+    // electronic state, as the rotational level the children of the
+    // vibrational state.
+    std::vector<int> find(const RawState& r) const {
+        std::vector<int> out;
+        auto gasOf = [&](const State& s) { return gases[s.gas].name; };
+        if (r.eleLevel == "*") {
+            for (size_t i = 0; i < states.size(); ++i)
+                if (gasOf(states[i]) == r.gas && states[i].type == "ele") {
+                    out.push_back((int)i);
+                    out.insert(out.end(), states[i].siblings.begin(), states[i].siblings.end());
+                    break;
+                }
+        } else if (r.vibLevel == "*") {
+            for (const State& s : states)
+                if (gasOf(s) == r.gas && s.raw.eleLevel == r.eleLevel && s.type == "ele") { out = s.children; break; }
+        } else if (r.rotLevel == "*") {
+            for (const State& s : states)
+                if (gasOf(s) == r.gas && s.raw.eleLevel == r.eleLevel && s.raw.vibLevel == r.vibLevel && s.type == "vib") {
+                    out = s.children;
+                    break;
+                }
+        } else {
+            const int i = findExact(r);
+            if (i >= 0) out.push_back(i);
+        }
+        return out;
+    }
+    std::string description(const Collision& c) const {  // Collision.C:88-122
+        std::string s = "e+" + states[c.target].raw.name();
+        s += c.isReverse ? "<->" : "->";
+        if (c.type == "Ionization") s += "e+e+";
+        else if (c.type != "Attachment") s += "e+";
+        for (size_t i = 0; i < c.products.size(); ++i) {
+            if (c.stoi[i] > 1) s += std::to_string((int)c.stoi[i]);
+            s += states[c.products[i]].raw.name();
+            if (i + 1 < c.products.size()) s += "+";
+        }
+        return s + "," + c.type;
+    }
+    // Collision::find: same type, threshold, direction and products among the
+    // target's collisions.
+    int findCollision(const Collision& c) const {
+        for (int k : states[c.target].colls) {
+            const Collision& o = colls[k];
+            if (o.threshold != c.threshold || o.type != c.type || o.isReverse != c.isReverse ||
+                o.products.size() != c.products.size())
+                continue;
+            bool equal = true;
+            for (size_t j = 0; j < o.products.size() && equal; ++j) {
+                bool any = false;
+                for (size_t q = 0; q < c.products.size(); ++q)
+                    if (o.products[j] == c.products[q] && o.stoi[j] == c.stoi[q]) { any = true; break; }
+                equal = any;
+            }
+            if (equal) return k;
+        }
+        return -1;
+    }
+    // Collision::add: a collision met again supplies its missing cross section
+    // (integral or momentum-transfer) or is dropped as a duplicate.
+    void addCollision(Collision c) {
+        const int k = findCollision(c);
+        if (k >= 0) {
+            Collision& o = colls[k];
+            if (!o.integral.defined() && c.integral.defined()) o.integral = c.integral;
+            else if (!o.momTransf.defined() && c.momTransf.defined()) o.momTransf = c.momTransf;
+            else warn("Warning! Avoiding duplicated electron impact collision:\n\t " + description(o));
+            return;
+        }
+        auto check = [&](XSec& x) {
+            for (size_t i = 0; i + 1 < x.e.size(); ++i)
+                if (x.e[i] >= x.e[i + 1])
+                    throw InputError("The energy column of the cross section corresponding to the collision shown below is not strictly increasing.\n" +
+                                     description(c) + "\nPlease fix this in the LXCat files.");
+            if ((c.type == "Elastic" || c.type == "Effective") && x.e[0] != 0) {
+                x.e.insert(x.e.begin(), 0.0);
+                x.s.insert(x.s.begin(), x.s[0]);
+            }
+        };
+        if (c.integral.defined()) check(c.integral);
+        if (c.momTransf.defined()) check(c.momTransf);
+        const int id = (int)colls.size();
+        colls.push_back(c);
+        if (!c.extra) {
+            gases[states[c.target].gas].colls.push_back(id);
+            states[c.target].colls.push_back(id);
+            states[c.target].isTarget = true;
+        }
+        if (c.isReverse) {
+            if (c.products.size() == 1 && c.stoi[0] == 1) {
+                if (!c.extra) {
+                    states[c.products[0]].colls.push_back(id);
+                    states[c.products[0]].isTarget = true;
+                }
+            } else {
+                throw InputError("Error while creating collision '" + description(c) +
+                                 "'. Klein-Rosseland microreversibility relation valid only for binary collisions.\n");
+            }
+        }
+        if ((c.type == "Effective" || c.type == "Elastic") && states[c.target].type != "ele")
+            throw InputError("Found ''" + c.type + "'' collision with ''" + states[c.target].raw.name() + "'' as target [" +
+                             description(c) + "].\n" + c.type +
+                             " collisions are only allowed for electronic states. Please check LXCat files.\n");
+    }
+    void addEntries(const std::vector<LXCatEntry>& entries, bool extra) {  // Setup.h:292-349
+        for (const LXCatEntry& e : entries) {
+            Collision c;
+            c.type = e.type;
+            c.target = addState(e.target);
+            for (const RawState& p : e.products) c.products.push_back(addState(p));
+            c.stoi = e.productStoi;
+            c.isReverse = e.isReverse;
+            c.threshold = e.threshold;
+            c.extra = extra;
+            XSec x;
+            x.e = e.energy;
+            x.s = e.sigma;
+            if (e.subtype == "momentum-transfer") c.momTransf = x;
+            else c.integral = x;
+            addCollision(c);
+        }
+    }
+};
+
+// Linear interpolation as gsl_interp_linear evaluates it: the interval with
+// x[i] <= v < x[i+1], the last point in the last interval.
+double linearAt(const std::vector<double>& x, const std::vector<double>& y, double v) {
+    if (x.size() < 2) return x.size() == 1 && v == x[0] ? y[0] : 0.0;
+    size_t lo = 0, hi = x.size() - 1;
+    while (hi > lo + 1) {
+        const size_t mid = (lo + hi) / 2;
+        if (x[mid] > v) hi = mid; else lo = mid;
+    }
+    const double dx = x[lo + 1] - x[lo];
+    return dx > 0.0 ? y[lo] + (v - x[lo]) / dx * (y[lo + 1] - y[lo]) : 0.0;
 }
 
-std::string descriptionOf(const Collision& c, const std::vector<State>& states) {
-    std::string s = "e+" + states[c.target].raw.name();
-    s += c.e.isReverse ? "<->" : "->";
-    if (c.e.type == "Ionization") s += "e+e+";
-    else if (c.e.type != "Attachment") s += "e+";
-    for (size_t i = 0; i < c.e.products.size(); ++i) {
-        if (c.e.productStoi[i] > 1) s += std::to_string((int)c.e.productStoi[i]);
-        s += c.e.products[i].name();
-        if (i + 1 < c.e.products.size()) s += "+";
+// Collision::interpolatedCrossSection (Collision.C:231-289): zero at or below
+// the threshold of an inelastic collision and outside the table.
+std::vector<double> interpolated(const Collision& c, bool momTransf, const std::vector<double>& energies) {
+    std::vector<double> out(energies.size(), 0.0);
+    size_t minIndex = 0;
+    if (c.type != "Effective" && c.type != "Elastic") {
+        minIndex = energies.size();
+        for (size_t i = 0; i < energies.size(); ++i)
+            if (energies[i] > c.threshold) { minIndex = i; break; }
     }
-    return s + "," + c.e.type;
+    const XSec& x = momTransf ? c.momTransf : c.integral;
+    for (size_t i = minIndex; i < energies.size(); ++i) {
+        const double v = energies[i];
+        out[i] = (v >= x.e.front() && v <= x.e.back()) ? linearAt(x.e, x.s, v) : 0.0;
+    }
+    return out;
+}
+
+// Collision::superElasticCrossSection (Collision.C:291-353), Klein-Rosseland.
+std::vector<double> superElastic(const Model& M, const Collision& c, bool momTransf, const std::vector<double>& energies) {
+    const State& t = M.states[c.target];
+    const State& p = M.states[c.products[0]];
+    if (!c.isReverse) throw InputError("Collision '" + M.description(c) + " is not defined as bidirectional.\n");
+    if (t.statWeight == kNonDef)
+        throw InputError("The statistical weight of the state '" + t.raw.name() + "' is not defined.\nThe super elastic cross section of the collision '" +
+                         M.description(c) + " cannot be evaluated.\n");
+    if (p.statWeight == kNonDef)
+        throw InputError("The statistical weight of the state '" + p.raw.name() + "' is not defined.\nThe super elastic cross section of the collision '" +
+                         M.description(c) + " cannot be evaluated.\n");
+    std::vector<double> out(energies.size(), 0.0);
+    if (energies.empty()) return out;
+    size_t minIndex = 0;
+    if (energies[0] == 0) {
+        if (energies.size() == 1) return out;
+        minIndex = 1;
+    }
+    std::vector<double> shifted(energies.size());
+    for (size_t i = 0; i < energies.size(); ++i) shifted[i] = energies[i] + c.threshold;
+    std::vector<double> in = interpolated(c, momTransf, shifted);
+    const double ratio = t.statWeight / p.statWeight;
+    for (double& v : in) v *= ratio;
+    for (size_t i = minIndex; i < energies.size(); ++i) out[i] = (1.0 + c.threshold / energies[i]) * in[i];
+    return out;
+}
+
+// AngularDistributionFunctions.h:32-105: I(energy, angle), normalized as
+// 2 pi Int_0^pi I sin(angle) d(angle) = 1.
+double angularPdf(const Collision& c, double energy, double angle) {
+    const double iso = 1.0 / (4.0 * kPi);
+    if (c.scattering == "bornDipole") {
+        const double after = energy - c.threshold;
+        if (after <= 0) return iso;
+        const double q2 = after + energy - 2.0 * std::sqrt(energy * after) * std::cos(angle);
+        const double se = std::sqrt(energy), sa = std::sqrt(after);
+        return iso * se * sa / q2 / std::log((sa + se) / std::sqrt(c.threshold));
+    }
+    if (c.scattering == "surendra") {
+        if (energy == 0) return iso;
+        return energy / (4.0 * kPi * (1.0 + energy * std::pow(std::sin(angle / 2.0), 2)) * std::log(1.0 + energy));
+    }
+    if (c.scattering == "coulombScreen") {
+        const double eps = c.params[0] == 0 ? energy : energy - c.threshold;
+        if (eps <= 0) return iso;
+        const double sp = c.params[1] / eps;
+        return (sp * (sp + 1.0)) / kPi / std::pow(2.0 * sp + 1.0 - std::cos(angle), 2);
+    }
+    if (c.scattering == "forward") return angle == 0 ? 1.0 : 0.0;
+    return iso;  // isotropic, momentumConservationIonization
+}
+
+// Collision::evaluateRawMomTransfCrossSection (Collision.C:124-196): the
+// missing one of the integral and momentum-transfer cross sections, from the
+// angular distribution (trapezoidal rule over `angleNumber` angles).
+void completeCrossSections(const Model& M, Collision& c, double angleNumber) {
+    const bool hasI = c.integral.defined(), hasM = c.momTransf.defined();
+    if (hasI && hasM) return;
+    if (c.scattering == "forward") {
+        if (!hasI)
+            throw InputError("When choosing the ''forward'' angularScatteringType for the collision\n" + M.description(c) +
+                             "\nthe integralCrossSection must be defined in the LXCat files, and not the momentum-transfer!");
+        c.momTransf.e = c.integral.e;
+        c.momTransf.s.assign(c.integral.e.size(), 0.0);
+        return;
+    }
+    if (c.scattering == "isotropic") {
+        if (hasI) c.momTransf = c.integral;
+        else c.integral = c.momTransf;
+        return;
+    }
+    const int n = (int)angleNumber;
+    if (n < 2) throw InputError("electronKinetics.anisotropicScattering.angleNumber must be at least 2.");
+    const double step = kPi / (angleNumber - 1.0);
+    std::vector<double> angles(n), aux(n);
+    for (int j = 0; j < n; ++j) {
+        angles[j] = j == n - 1 ? kPi : (double)j * step;
+        aux[j] = std::cos(angles[j]) * std::sin(angles[j]);
+    }
+    XSec& from = hasI ? c.integral : c.momTransf;
+    XSec& to = hasI ? c.momTransf : c.integral;
+    to.e = from.e;
+    to.s.assign(from.e.size(), 0.0);
+    for (size_t i = 0; i < from.e.size(); ++i) {
+        const double energy = from.e[i];
+        double a = aux[0] * angularPdf(c, energy, angles[0]);
+        for (int j = 1; j < n - 1; ++j) a += 2.0 * aux[j] * angularPdf(c, energy, angles[j]);
+        a += aux[n - 1] * angularPdf(c, energy, angles[n - 1]);
+        a *= kPi * step;
+        to.s[i] = hasI ? from.s[i] * (1.0 - a) : from.s[i] / (1.0 - a);
+    }
+}
+
+// ----- state property functions (StatePropertyFunctions.h:22-866)
+
+double temperatureArgument(const std::vector<std::string>& strArgs, const std::vector<double>& args, size_t i,
+                           double gasTemperature) {
+    if (strArgs[i] == "gasTemperature") return gasTemperature;
+    if (strArgs[i] == "electronTemperature")
+        throw InputError("Property functions of the electron temperature are not supported by this program (the Monte Carlo "
+                         "electron kinetics has no prescribed electron temperature).");
+    return args[i];
+}
+
+double levelNumber(const std::string& s) { return std::stod(s); }
+
+void boltzmannLike(Model& M, const std::vector<int>& ids, double T, double cutoff, int cutoffLevel) {
+    double ground = 1e20, norm = 0;
+    for (int i : ids) ground = std::fmin(ground, M.states[i].energy);
+    for (int i : ids) {
+        State& s = M.states[i];
+        const double level = cutoffLevel == 1 ? levelNumber(s.raw.vibLevel) : (cutoffLevel == 2 ? levelNumber(s.raw.rotLevel) : 0);
+        if (cutoffLevel != 0 && level > cutoff) {
+            s.population = 0;
+        } else {
+            s.population = s.statWeight * std::exp(-(s.energy - ground) / (kBoltzmannInEV * T));
+            norm += s.population;
+        }
+    }
+    for (int i : ids) M.states[i].population /= norm;
+}
+
+void treanor(Model& M, const std::vector<int>& ids, double T0, double T1, bool gordiets, const std::string& fname) {
+    double groundE = 0, firstE = 0;
+    bool g0 = false, g1 = false;
+    for (int i : ids) {
+        const State& s = M.states[i];
+        if (s.raw.vibLevel == "0") { groundE = s.energy; g0 = true; }
+        else if (s.raw.vibLevel == "1") { firstE = s.energy; g1 = true; }
+    }
+    if (!g0 || !g1)
+        throw InputError("Unable to find groundEnergy or firstEnergy to populate state" + M.states[ids[0]].raw.name() +
+                         " and its siblings with function " + fname + ". \nCheck input file.");
+    double norm = 0;
+    for (int i : ids) {
+        State& s = M.states[i];
+        const double v = levelNumber(s.raw.vibLevel);
+        s.population = s.statWeight * std::exp(-1.0 / kBoltzmannInEV *
+                                               (v * (firstE - groundE) * (1.0 / T1 - 1.0 / T0) + (s.energy - groundE) / T0));
+        norm += s.population;
+    }
+    for (int i : ids) M.states[i].population /= norm;
+    if (!gordiets) return;
+    const double anh = M.gases[M.states[ids[0]].gas].anharmonicFrequency;
+    const double vLimit = std::floor(0.5 * (1.0 + (firstE - groundE) * T0 / (kPlanckReducedInEV * anh * T1)));
+    int atLimit = -1;
+    for (size_t k = 0; k < ids.size(); ++k)
+        if (levelNumber(M.states[ids[k]].raw.vibLevel) == vLimit) atLimit = (int)k;
+    if (atLimit < 0)
+        throw InputError("treanorGordietsPopulation of " + M.states[ids[0]].raw.name() +
+                         ": no vibrational state at the Treanor minimum v = " + std::to_string((int)vLimit) + ".");
+    std::vector<double> tg(ids.size(), 0.0);
+    norm = 0;
+    for (size_t k = 0; k < ids.size(); ++k) {
+        const double v = levelNumber(M.states[ids[k]].raw.vibLevel);
+        tg[k] = v <= vLimit ? M.states[ids[k]].population : M.states[ids[atLimit]].population * vLimit / v;
+        norm += tg[k];
+    }
+    for (size_t k = 0; k < ids.size(); ++k) M.states[ids[k]].population = tg[k] / norm;
+}
+
+void evaluateStateProperty(Model& M, const std::string& property, const std::vector<int>& ids, const std::string& value,
+                           const std::map<std::string, std::string>& workCond, double gasTemperature) {
+    const std::string func = splitChars(value, "@").empty() ? value : splitChars(value, "@")[0];
+    std::vector<double> args;
+    std::vector<std::string> strArgs;
+    if (value.find('@') != std::string::npos) {  // Setup.h:561-574
+        const std::vector<std::string> parts = splitChars(value, "@");
+        if (parts.size() > 1) strArgs = splitChars(parts[1], ",");
+        for (const std::string& a : strArgs) {
+            if (a == "gasTemperature") args.push_back(gasTemperature);
+            else if (workCond.count(a)) args.push_back(evalNumber(workCond.at(a)));
+            else args.push_back(evalNumber(a));
+        }
+    }
+    State& first = M.states[ids[0]];
+    const Gas& gas = M.gases[first.gas];
+    auto need = [&](bool ok, const std::string& msg) { if (!ok) throw InputError(msg); };
+    auto noArgs = [&](const std::string& f) {
+        need(args.empty(), "Wrong number of arguments when evaluating " + f + " function. \nCheck input file");
+    };
+    auto forProperty = [&](const std::string& f, const char* prop) {
+        need(property == prop, "Trying to use " + f + " function to set up property " + property + ". \nCheck input file");
+    };
+    auto checkEnergiesWeights = [&](const std::string& f) {
+        for (int i : ids) {
+            const State& s = M.states[i];
+            need(s.energy != kNonDef, "Unable to find " + s.raw.name() + " energy for the evaluation of " + f + " function. \nCheck input file");
+            need(s.statWeight != kNonDef, "Unable to find " + s.raw.name() + " statistical weight for the evaluation of " + f +
+                                               " function. \nCheck input file\n");
+        }
+    };
+    if (func.rfind("rotationalDegeneracy", 0) == 0 &&
+        (func == "rotationalDegeneracy" || func == "rotationalDegeneracy_H2" || func == "rotationalDegeneracy_N2" ||
+         func == "rotationalDegeneracy_H2O" || func == "rotationalDegeneracy_NO")) {
+        forProperty(func, "statisticalWeight");
+        noArgs(func);
+        need(first.type == "rot", "Trying to assign " + func + " to non rotational state " + first.raw.name() + ". \nCheck input file");
+        for (int i : ids) {
+            State& s = M.states[i];
+            if (func == "rotationalDegeneracy_H2O") {
+                const double J = std::stod(s.raw.rotLevel.substr(0, 1)), Ka = std::stod(s.raw.rotLevel.substr(1, 1)),
+                             Kc = std::stod(s.raw.rotLevel.substr(2, 1));
+                double w = 2.0 * J + 1;
+                if (std::fmod(std::fabs(Ka - Kc), 2) != 0) w *= 3.0;
+                s.statWeight = w;
+                continue;
+            }
+            const double J = levelNumber(s.raw.rotLevel);
+            if (func == "rotationalDegeneracy") s.statWeight = 2.0 * J + 1.0;
+            else if (func == "rotationalDegeneracy_H2") s.statWeight = (2.0 - std::pow(-1, J)) * (2.0 * J + 1.0);
+            else if (func == "rotationalDegeneracy_N2") s.statWeight = 3.0 * (1.0 + 0.5 * (1.0 + std::pow(-1, J))) * (2.0 * J + 1.0);
+            else s.statWeight = 3.0 * (2.0 * J + 1.0);  // NO
+        }
+    } else if (func == "harmonicOscillatorEnergy" || func == "morseOscillatorEnergy") {
+        forProperty(func, "energy");
+        noArgs(func);
+        need(first.type == "vib", "Trying to assign " + func + " to non vibrational state " + first.raw.name() + ".\nCheck input file.");
+        need(gas.harmonicFrequency != kNonDef, "Unable to find harmonicFrequency to evaluate the energy of the state " +
+                                                   first.raw.name() + " with function " + func + ".\nCheck input file");
+        if (func == "morseOscillatorEnergy")
+            need(gas.anharmonicFrequency != kNonDef, "Unable to find anharmonicFrequency to evaluate the energy of the state " +
+                                                         first.raw.name() + " with function morseOscillatorEnergy.\nCheck input file");
+        for (int i : ids) {
+            State& s = M.states[i];
+            const double v = levelNumber(s.raw.vibLevel);
+            const Gas& sg = M.gases[s.gas];
+            s.energy = func == "harmonicOscillatorEnergy"
+                           ? kPlanckReducedInEV * sg.harmonicFrequency * (v + 0.5)
+                           : kPlanckReducedInEV * (sg.harmonicFrequency * (v + 0.5) - sg.anharmonicFrequency * std::pow((v + 0.5), 2));
+        }
+    } else if (func == "rigidRotorEnergy" || func == "rigidRotorEnergy_NO") {
+        forProperty(func, "energy");
+        noArgs(func);
+        need(first.type == "rot", "Trying to assign " + func + " to non rotational state " + first.raw.name() + "\nCheck input file.");
+        if (func == "rigidRotorEnergy")
+            need(gas.rotationalConstant != kNonDef, "Unable to find rotationalConstant to evaluate the energy of the state " +
+                                                        first.raw.name() + " with function rigidRotorEnergy.\nCheck input file");
+        const bool x12 = first.raw.eleLevel == "X_1/2";
+        for (int i : ids) {
+            State& s = M.states[i];
+            const double J = levelNumber(s.raw.rotLevel);
+            if (func == "rigidRotorEnergy") {
+                s.energy = M.gases[s.gas].rotationalConstant * J * (J + 1.0);
+            } else {
+                const double B = x12 ? 0.2073e-3 : 0.2133e-3, omega = x12 ? 0.5 : 1.5;
+                s.energy = B * (J * (J + 1.0) - omega * omega);
+            }
+        }
+    } else if (func == "boltzmannPopulation" || func == "boltzmannPopulationVibrationalCutoff" ||
+               func == "boltzmannPopulationRotationalCutoff") {
+        forProperty(func, "population");
+        const int cut = func == "boltzmannPopulation" ? 0 : (func == "boltzmannPopulationVibrationalCutoff" ? 1 : 2);
+        need(args.size() == (cut == 0 ? 1u : 2u), "Wrong number of arguments when evaluating " + func + " function. \nCheck input file");
+        if (cut == 1) need(first.type == "vib", "Trying to assign " + func + " to non vibrational state" + first.raw.name() + ". \nCheck input file");
+        if (cut == 2) need(first.type == "rot", "Trying to assign " + func + " to non rotational state" + first.raw.name() + ". \nCheck input file");
+        if (cut != 0)
+            need(std::fmod(args[1], 1) == 0 && args[1] >= 0,
+                 "Error found when evaluating population of state " + first.raw.name() + ".\nMaximum " +
+                     (cut == 1 ? "vibrational" : "rotational") + " level ''" + std::to_string(args[1]) +
+                     "should be a non-negative integer.\nCheck input file");
+        checkEnergiesWeights(func);
+        boltzmannLike(M, ids, temperatureArgument(strArgs, args, 0, gasTemperature), cut ? args[1] : 0, cut);
+    } else if (func == "treanorPopulation" || func == "treanorGordietsPopulation") {
+        forProperty(func, "population");
+        need(args.size() == 2, "Wrong number of arguments when evaluating " + func + " function. \nCheck input file");
+        if (func == "treanorGordietsPopulation")
+            need(gas.anharmonicFrequency != kNonDef, "Unable to find " + gas.name + " anharmonicFrequency to populate" +
+                                                         first.raw.name() + " and its siblings with function treanorGordietsPopulation. \nCheck input file\n");
+        need(first.type == "vib", "Trying to assign " + func + " to non vibrational state" + first.raw.name() + ". \nCheck input file");
+        checkEnergiesWeights(func);
+        treanor(M, ids, temperatureArgument(strArgs, args, 0, gasTemperature), temperatureArgument(strArgs, args, 1, gasTemperature),
+                func == "treanorGordietsPopulation", func);
+    } else if (value.find('@') != std::string::npos) {
+        throw InputError("Error! Trying to use the property function '" + func + "' which is not defined in the code.\n");
+    } else {
+        const double v = evalNumber(value);
+        for (int i : ids) {
+            State& s = M.states[i];
+            if (property == "energy") s.energy = v;
+            else if (property == "statisticalWeight") s.statWeight = v;
+            else s.population = v;
+        }
+    }
+}
+
+// EedfGas::checkPopulationNorms (EedfGas.C:29-101)
+void checkPopulationNorms(const Model& M, const Gas& g) {
+    if (g.fraction == 0) return;
+    double gasNorm = 0;
+    bool eleDone = false, ionDone = false;
+    auto err = [](double x) { return std::fabs(x - 1) > 10 * kEps; };
+    for (int si : g.states) {
+        const State& st = M.states[si];
+        if (st.type == "ele" && !eleDone) {
+            std::vector<int> group(1, si);
+            group.insert(group.end(), st.siblings.begin(), st.siblings.end());
+            for (int ei : group) {
+                const State& e = M.states[ei];
+                if (!(e.isTarget && e.population != 0)) continue;
+                gasNorm += e.population;
+                if (e.children.empty()) continue;
+                double vibNorm = 0;
+                bool anyVib = false;
+                for (int vi : e.children) {
+                    const State& v = M.states[vi];
+                    if (!(v.isTarget && v.population != 0)) continue;
+                    anyVib = true;
+                    vibNorm += v.population;
+                    if (v.children.empty()) continue;
+                    double rotNorm = 0;
+                    for (int ri : v.children) {
+                        const State& r = M.states[ri];
+                        if (r.isTarget && r.population != 0) rotNorm += r.population;
+                    }
+                    if (err(rotNorm))
+                        throw InputError("Rotational distribution " + v.raw.name().substr(0, v.raw.name().size() - 1) +
+                                         ",J=*) is not properly normalized. (Error = " + std::to_string(rotNorm - 1) + ")\n");
+                }
+                if (anyVib && err(vibNorm))
+                    throw InputError("Vibrational distribution " + e.raw.name().substr(0, e.raw.name().size() - 1) +
+                                     ",v=*) is not properly normalized. (Error = " + std::to_string(vibNorm - 1) + ")\n");
+            }
+            eleDone = true;
+        }
+        if (st.type == "ion" && !ionDone) {
+            std::vector<int> group(1, si);
+            group.insert(group.end(), st.siblings.begin(), st.siblings.end());
+            for (int ii : group)
+                if (M.states[ii].population != 0) gasNorm += M.states[ii].population;
+            ionDone = true;
+        }
+    }
+    if (err(gasNorm))
+        throw InputError("Electronic/ionic distribution " + g.name + "(*) is not properly normalized. (Error = " +
+                         std::to_string(gasNorm - 1) + ")\n");
+}
+
+// EedfGas::elasticFromEffectiveCrossSection (EedfGas.C:166-301): the
+// Effective cross section minus every other collision of the gas weighted by
+// the population of its target (default: ground electronic state, Boltzmann
+// vibrational and rotational distributions at 300 K).
+XSec elasticFromEffective(Model& M, int gi) {
+    Gas& g = M.gases[gi];
+    int eff = -1;
+    for (int k : g.colls)
+        if (M.colls[k].type == "Effective") { eff = k; break; }
+    if (eff < 0)
+        throw InputError("Gas ''" + g.name + "'' does not have an ''Effective'' collision defined, so ''Elastic'' collisions cannot be evaluated from it.\nPlease, check the corresponding LXCat file.");
+    XSec raw = M.colls[eff].momTransf;
+    if (!g.effPopDefined) {
+        g.effPopDefined = true;
+        const int ground = M.colls[eff].target;
+        g.effPop[ground] = 1;
+        const State& gs = M.states[ground];
+        int vibGround = -1;
+        auto check = [&](const State& s) {
+            if (s.energy == kNonDef)
+                throw InputError("Unable to find " + s.raw.name() + " energy for the evaluation of ''Elastic'' cross section of " + g.name + ".\nCheck input file");
+            if (s.statWeight == kNonDef)
+                throw InputError("Unable to find " + s.raw.name() + " statistical weight for the evaluation of ''Elastic'' cross section of " + g.name + ".\nCheck input file");
+        };
+        if (!gs.children.empty()) {
+            double norm = 0;
+            vibGround = gs.children[0];
+            for (int c : gs.children) {
+                const State& s = M.states[c];
+                check(s);
+                if (s.energy < M.states[vibGround].energy) vibGround = c;
+                g.effPop[c] = s.statWeight * std::exp(-s.energy / (kBoltzmannInEV * 300.0));
+                norm += g.effPop[c];
+            }
+            for (int c : gs.children) g.effPop[c] = g.effPop[c] / norm;
+        }
+        if (vibGround >= 0 && !M.states[vibGround].children.empty()) {
+            double norm = 0;
+            for (int c : M.states[vibGround].children) {
+                const State& s = M.states[c];
+                check(s);
+                g.effPop[c] = s.statWeight * std::exp(-s.energy / (kBoltzmannInEV * 300.0));
+                norm += g.effPop[c];
+            }
+            for (int c : M.states[vibGround].children) g.effPop[c] = g.effPop[vibGround] * g.effPop[c] / norm;
+        }
+    }
+    auto pop = [&](int state) {
+        const auto it = g.effPop.find(state);
+        return it == g.effPop.end() ? 0.0 : it->second;
+    };
+    for (int k : g.colls) {
+        const Collision& c = M.colls[k];
+        if (c.type == "Effective" || c.type == "Elastic") continue;
+        const std::vector<double> in = interpolated(c, true, raw.e);
+        for (size_t i = 0; i < raw.s.size(); ++i) raw.s[i] -= pop(c.target) * in[i];
+        if (c.isReverse) {
+            const std::vector<double> sup = superElastic(M, c, true, raw.e);
+            for (size_t i = 0; i < raw.s.size(); ++i) raw.s[i] -= pop(c.products[0]) * sup[i];
+        }
+    }
+    bool negative = false;
+    for (double& v : raw.s)
+        if (v < 0) { negative = true; v = 0; }
+    if (negative)
+        warn("Negative values obtained when evaluating an Elastic cross section from an Effective one (" + g.name +
+             ").\nNegative values have been clipped to 0 and unreliable results may be obtained.\nPlease, carefully check inputs and outputs of your simulation");
+    return raw;
+}
+
+// EedfGas::checkElasticCollisions (EedfGas.C:120-164)
+void checkElasticCollisions(Model& M, int gi) {
+    const Gas& g = M.gases[gi];
+    if (g.colls.empty()) return;
+    for (int si : g.states) {
+        if (M.states[si].type != "ele") continue;
+        std::vector<int> group(1, si);
+        group.insert(group.end(), M.states[si].siblings.begin(), M.states[si].siblings.end());
+        for (int ei : group) {
+            if (!M.states[ei].isTarget) continue;
+            bool hasElastic = false;
+            for (int k : M.states[ei].colls) hasElastic |= M.colls[k].type == "Elastic";
+            if (hasElastic) continue;
+            const XSec raw = elasticFromEffective(M, gi);
+            Collision c;
+            c.type = "Elastic";
+            c.target = ei;
+            c.products.assign(1, ei);
+            c.stoi.assign(1, 1.0);
+            c.integral = raw;
+            c.momTransf = raw;
+            M.addCollision(c);
+        }
+        break;
+    }
 }
 
 }  // namespace
 
-GasMixture buildMixture(const SetupFile& setup, const std::string& inputDir) {
+GasMixture buildMixture(const SetupFile& setup, const std::string& inputDir, double gasTemperature) {
+    Model M;
+    // gases, states and collisions of the LXCat files (Setup.h:292-349)
     const std::vector<LXCatEntry> entries = readLXCat(setup.childNames("electronKinetics.LXCatFiles"), inputDir);
     if (entries.empty()) throw InputError("No LXCat collisions found in electronKinetics.LXCatFiles.");
-    // gases and states in order of first appearance (Setup.h:300-320)
-    std::vector<std::string> gases;
-    std::vector<State> states;
-    auto gasId = [&](const std::string& g) {
-        for (size_t i = 0; i < gases.size(); ++i)
-            if (gases[i] == g) return (int)i;
-        gases.push_back(g);
-        return (int)gases.size() - 1;
-    };
-    auto stateId = [&](const RawState& r) {
-        const int g = gasId(r.gas);
-        for (size_t i = 0; i < states.size(); ++i)
-            if (states[i].raw == r) return (int)i;
-        State s;
-        s.raw = r;
-        s.gas = g;
-        states.push_back(s);
-        return (int)states.size() - 1;
-    };
-    std::vector<Collision> colls;
-    for (const LXCatEntry& e : entries) {
-        Collision c;
-        c.e = e;
-        c.target = stateId(e.target);
-        c.gas = states[c.target].gas;
-        for (const RawState& p : e.products) {
-            const int id = stateId(p);
-            if (c.product < 0) c.product = id;
-        }
-        if (e.isReverse && (e.products.size() != 1 || e.productStoi[0] != 1))
-            throw InputError("Error while creating collision '" + descriptionOf(c, states) +
-                             "'. Klein-Rosseland microreversibility relation valid only for binary collisions.\n");
-        if ((e.type == "Effective" || e.type == "Elastic") && !(e.target.vibLevel.empty() && e.target.rotLevel.empty()))
-            throw InputError("Found ''" + e.type + "'' collision with ''" + e.target.name() + "'' as target [" +
-                             descriptionOf(c, states) + "].\n" + e.type + " collisions are only allowed for electronic states. Please check LXCat files.\n");
-        colls.push_back(c);
-    }
-    // gas properties
-    const int nG = (int)gases.size();
-    std::vector<double> mass(nG, -1), fraction(nG, 0), opb(nG, -1);
-    std::vector<bool> hasColl(nG, false);
-    for (const Collision& c : colls) hasColl[c.gas] = true;
-    auto gasMap = [&](const std::string& key, std::vector<double>& dst, const char* what) {
-        for (const auto& kv : setup.propertyMap("electronKinetics.gasProperties." + key, inputDir))
-            for (int g = 0; g < nG; ++g)
-                if (gases[g] == kv.first) dst[g] = propertyNumber(kv.first, kv.second, what);
-    };
-    gasMap("mass", mass, "mass");
-    gasMap("fraction", fraction, "fraction");
-    gasMap("OPBParameter", opb, "OPB parameter");
-    // state properties
-    for (const auto& kv : setup.propertyMap("electronKinetics.stateProperties.population", inputDir)) {
-        if (kv.first.find('*') != std::string::npos)
-            throw InputError("Population of '" + kv.first + "': wildcard states are not supported by this reader.");
-        const RawState r = parseState(kv.first);
-        for (State& s : states)
-            if (s.raw == r) { s.population = propertyNumber(kv.first, kv.second, "population"); s.popSet = true; }
-    }
-    for (const auto& kv : setup.propertyMap("electronKinetics.stateProperties.statisticalWeight", inputDir)) {
-        const RawState r = parseState(kv.first);
-        for (State& s : states)
-            if (s.raw == r) s.statWeight = propertyNumber(kv.first, kv.second, "statistical weight");
-    }
-    // densities (State.h:72-85): electronic states carry the gas fraction,
-    // vibrational and rotational ones their parents' populations
-    auto parentPop = [&](const RawState& r, bool vibParent) {
-        RawState p = r;
-        p.rotLevel.clear();
-        if (!vibParent) p.vibLevel.clear();
-        for (const State& s : states)
-            if (s.raw == p) return s.population;
-        return 0.0;
-    };
-    std::vector<double> density(states.size(), 0);
-    for (size_t i = 0; i < states.size(); ++i) {
-        const State& s = states[i];
-        double d = s.population * fraction[s.gas];
-        if (!s.raw.rotLevel.empty()) d *= parentPop(s.raw, true) * parentPop(s.raw, false);
-        else if (!s.raw.vibLevel.empty()) d *= parentPop(s.raw, false);
-        density[i] = d;
-    }
-    for (int g = 0; g < nG; ++g) {
-        if (!hasColl[g]) continue;
-        if (mass[g] < 0)
-            throw InputError("Mass of gas " + gases[g] +
-                             " not found.\nNeeded for the evaluation of the elastic collision operator (Boltzmann).\nCheck input file.");
-        double norm = 0;
-        bool anyPop = false;
-        for (const State& s : states)
-            if (s.gas == g && s.raw.vibLevel.empty() && s.raw.rotLevel.empty() && s.raw.ionCharg.empty()) {
-                norm += s.population;
-                anyPop |= s.popSet;
+    M.addEntries(entries, false);
+    if (setup.has("electronKinetics.LXCatFilesExtra"))
+        M.addEntries(readLXCat(setup.childNames("electronKinetics.LXCatFilesExtra"), inputDir), true);
+    M.fixOrphans();
+    // gas properties (Setup.h:462-511); heat capacities and conductivities do
+    // not enter the electron kinetics
+    auto gasProperty = [&](const char* key, double Gas::*field) {
+        for (const auto& kv : setup.propertyMap(std::string("electronKinetics.gasProperties.") + key, inputDir)) {
+            for (Gas& g : M.gases) {
+                if (g.name != kv.first) continue;
+                if (kv.second.find('@') != std::string::npos)
+                    throw InputError("Error! Trying to use the property function '" + splitChars(kv.second, "@")[0] +
+                                     "' which is not defined in the code.\n");
+                g.*field = evalNumber(kv.second);
             }
-        if (anyPop && std::fabs(norm - 1.0) > 10 * 2.220446049250313e-16)
-            throw InputError("Error: the electronic populations of gas " + gases[g] + " are not normalized.");
-        bool elastic = false, effective = false;
-        for (const Collision& c : colls)
-            if (c.gas == g) { elastic |= c.e.type == "Elastic"; effective |= c.e.type == "Effective"; }
-        if (!elastic && effective)
-            throw InputError("Gas " + gases[g] + " has an Effective cross section and no Elastic one; deriving the elastic cross section from the effective one is not supported by this reader.");
+        }
+    };
+    gasProperty("mass", &Gas::mass);
+    gasProperty("harmonicFrequency", &Gas::harmonicFrequency);
+    gasProperty("anharmonicFrequency", &Gas::anharmonicFrequency);
+    gasProperty("rotationalConstant", &Gas::rotationalConstant);
+    gasProperty("fraction", &Gas::fraction);
+    gasProperty("OPBParameter", &Gas::opb);
+    {
+        double norm = 0;
+        for (const Gas& g : M.gases) norm += g.fraction;
+        if (std::fabs(norm - 1) > 10 * kEps)
+            throw InputError("Gas fractions are not properly normalized (Error = " + std::to_string(norm - 1) +
+                             "). Please, check input file.\n");
     }
-    // anisotropic scattering (Setup.h:351-451)
+    // state properties (Setup.h:513-552), each map in key order
+    std::map<std::string, std::string> workCond;
+    for (const std::string& k : setup.childNames("workingConditions")) workCond[k] = setup.value("workingConditions." + k);
+    for (const char* prop : {"energy", "statisticalWeight", "population"}) {
+        for (const auto& kv : setup.propertyMap(std::string("electronKinetics.stateProperties.") + prop, inputDir)) {
+            const std::vector<int> ids = M.find(parseState(kv.first));
+            if (ids.empty()) continue;
+            evaluateStateProperty(M, prop, ids, kv.second, workCond, gasTemperature);
+        }
+    }
+    // densities (State.h:72-85)
+    for (State& s : M.states) {
+        const double f = M.gases[s.gas].fraction;
+        if (s.type == "rot") s.density = s.population * M.states[s.parent].population * M.states[M.states[s.parent].parent].population * f;
+        else if (s.type == "vib") s.density = s.population * M.states[s.parent].population * f;
+        else s.density = s.population * f;
+    }
+    // populations for an Elastic cross section derived from an Effective one
+    // (Setup.h:250-269)
+    {
+        std::map<std::string, std::string> eff;
+        if (setup.has("electronKinetics.effectiveCrossSectionPopulations"))
+            eff = setup.propertyMap("electronKinetics.effectiveCrossSectionPopulations", inputDir);
+        if (!eff.empty())
+            for (Gas& g : M.gases) g.effPopDefined = true;
+        for (const auto& kv : eff) {
+            const double v = evalNumber(kv.second);
+            for (int id : M.find(parseState(kv.first))) M.gases[M.states[id].gas].effPop[id] = v;
+        }
+    }
+    // anisotropic scattering (Setup.h:351-460)
+    double angleNumber = 0;
     if (setup.has("electronKinetics.anisotropicScattering") &&
         (setup.value("electronKinetics.anisotropicScattering.isOn") == "true" ||
          setup.value("electronKinetics.anisotropicScattering.isOn") == "True" ||
          setup.value("electronKinetics.anisotropicScattering.isOn") == "1")) {
+        angleNumber = evalNumber(setup.value("electronKinetics.anisotropicScattering.angleNumber"));
         std::vector<std::string> lines;
         for (const std::string& n : setup.childNames("electronKinetics.anisotropicScattering.collisions")) {
             if (n.find(';') != std::string::npos) { lines.push_back(n); continue; }
@@ -379,12 +1018,20 @@ GasMixture buildMixture(const SetupFile& setup, const std::string& inputDir) {
                     for (const std::string& t : splitChars(p[at], ",")) v.push_back(evalNumber(t));
                 return v;
             };
-            if (!p.empty() && p[0] == "group" && p.size() >= 4) {
-                for (Collision& c : colls)
-                    if (gases[c.gas] == p[1] && c.e.type == p[2]) { c.scattering = p[3]; c.params = params(4); found = true; }
-            } else if (!p.empty() && p[0] == "single" && p.size() >= 3) {
-                for (Collision& c : colls)
-                    if (descriptionOf(c, states) == p[1]) { c.scattering = p[2]; c.params = params(3); found = true; break; }
+            if (!p.empty() && p[0] == "group") {
+                if (p.size() < 4)
+                    throw InputError("Error while reading the following setup line of electronKinetics.anisotropicScattering.collisions:\n" + l);
+                for (const Gas& g : M.gases) {
+                    if (g.name != p[1]) continue;
+                    for (int k : g.colls)
+                        if (M.colls[k].type == p[2]) { M.colls[k].scattering = p[3]; M.colls[k].params = params(4); found = true; }
+                    break;
+                }
+            } else if (!p.empty() && p[0] == "single") {
+                if (p.size() < 3)
+                    throw InputError("Error while reading the following setup line of electronKinetics.anisotropicScattering.collisions:\n" + l);
+                for (Collision& c : M.colls)
+                    if (M.description(c) == p[1]) { c.scattering = p[2]; c.params = params(3); found = true; break; }
             } else {
                 throw InputError("Error while reading the following setup line of electronKinetics.anisotropicScattering.collisions:\n" + l);
             }
@@ -393,88 +1040,91 @@ GasMixture buildMixture(const SetupFile& setup, const std::string& inputDir) {
                                  l + "\nThe collision (group or single) was not found.");
         }
     }
-    for (const Collision& c : colls) {
-        scatteringCode(c.scattering);
-        if ((int)c.params.size() != requiredParams(c.scattering))
-            throw InputError("The angular scattering function '" + c.scattering + "' requires exactly " +
-                             std::to_string(requiredParams(c.scattering)) + " parameters, and not " + std::to_string(c.params.size()));
-        if (c.scattering != "isotropic" && (c.e.type == "Effective" || c.e.type == "Attachment"))
-            throw InputError("'" + c.e.type + "' collisions cannot have a user-prescribed angular scattering model.");
-        if (c.scattering != "isotropic" && c.e.subtype == "momentum-transfer")
-            throw InputError("Collision '" + descriptionOf(c, states) +
-                             "' has only a momentum-transfer cross section and an anisotropic scattering model; the angular conversion to an integral cross section is not supported by this reader.");
+    for (const Gas& g : M.gases)
+        for (int k : g.colls) {
+            Collision& c = M.colls[k];
+            scatteringCode(c.scattering);
+            if ((int)c.params.size() != requiredParams(c.scattering))
+                throw InputError("The angular scattering function '" + c.scattering + "' requires exactly " +
+                                 std::to_string(requiredParams(c.scattering)) + " parameters, and not " + std::to_string(c.params.size()));
+            completeCrossSections(M, c, angleNumber);
+            if (c.scattering != "isotropic" && (c.type == "Effective" || c.type == "Attachment"))
+                throw InputError("Error in the following collision:\n" + M.description(c) + "\n'" + c.type +
+                                 "' collisions cannot have a user-prescribed angular scattering model.");
+        }
+    // per-gas checks (Setup.h:271-289)
+    for (int gi = 0; gi < (int)M.gases.size(); ++gi) {
+        if (M.gases[gi].colls.empty()) continue;
+        if (M.gases[gi].mass == kNonDef)
+            throw InputError("Mass of gas " + M.gases[gi].name +
+                             " not found.\nNeeded for the evaluation of the elastic collision operator (Boltzmann).\nCheck input file.");
+        checkPopulationNorms(M, M.gases[gi]);
+        checkElasticCollisions(M, gi);
     }
     // flatten (BoltzmannMC.C:29-271): gases with collisions in order, their
     // non-Effective collisions in order, a superelastic entry after its partner
     GasMixture m;
-    m.allGasNames = gases;
+    for (const Gas& g : M.gases) m.allGasNames.push_back(g.name);
     m.tableOffset.push_back(0);
-    std::vector<int> gasSlot(nG, -1);
-    for (int g = 0; g < nG; ++g) {
-        if (!hasColl[g]) continue;
-        gasSlot[g] = (int)m.gasNames.size();
-        m.gasNames.push_back(gases[g]);
-        m.gasMass.push_back(mass[g]);
-        m.gasFraction.push_back(fraction[g]);
-        for (const Collision& c : colls) {
-            if (c.gas != g || c.e.type == "Effective") continue;
+    for (const Gas& g : M.gases) {
+        if (g.colls.empty()) continue;
+        const int slot = (int)m.gasNames.size();
+        m.gasNames.push_back(g.name);
+        m.gasMass.push_back(g.mass);
+        m.gasFraction.push_back(g.fraction);
+        for (int k : g.colls) {
+            const Collision& c = M.colls[k];
+            if (c.type == "Effective") continue;
             const int cid = (int)m.collisionDescription.size();
-            std::vector<double> en = c.e.energy, sg = c.e.sigma;
-            for (size_t i = 0; i + 1 < en.size(); ++i)
-                if (en[i] >= en[i + 1])
-                    throw InputError("The energy column of the cross section corresponding to the collision shown below is not strictly increasing.\n" +
-                                     descriptionOf(c, states) + "\nPlease fix this in the LXCat files.");
-            if (c.e.type == "Elastic" && en[0] != 0) {  // Collision.C:40-55
-                en.insert(en.begin(), 0.0);
-                sg.insert(sg.begin(), sg[0]);
+            const State& tgt = M.states[c.target];
+            const int prodId = c.products.empty() ? -1 : c.products[0];
+            const int code = c.type == "Ionization" ? EKMC_IONIZATION : (c.type == "Attachment" ? EKMC_ATTACHMENT : EKMC_CONSERVATIVE);
+            const std::string desc = M.description(c);
+            double ratio = 0;
+            if (c.isReverse) {
+                const double gt = tgt.statWeight, gp = M.states[prodId].statWeight;
+                if (gt == kNonDef || gp == kNonDef || gp == 0)
+                    throw InputError("Statistical weights of " + tgt.raw.name() + " and " + M.states[prodId].raw.name() +
+                                     " are needed for the superelastic collision of\n" + desc);
+                ratio = gt / gp;
             }
-            const int code = c.e.type == "Ionization" ? EKMC_IONIZATION : (c.e.type == "Attachment" ? EKMC_ATTACHMENT : EKMC_CONSERVATIVE);
-            const std::string desc = descriptionOf(c, states);
             auto push = [&](bool super) {
                 m.type.push_back(code);
-                m.isElastic.push_back(c.e.type == "Elastic" ? 1 : 0);
+                m.isElastic.push_back(c.type == "Elastic" ? 1 : 0);
                 m.isSuperelastic.push_back(super ? 1 : 0);
-                m.gasIndex.push_back(gasSlot[g]);
-                m.threshold.push_back(c.e.threshold);
-                m.targetMass.push_back(mass[g]);
-                m.relDensity.push_back(super ? density[c.product] : density[c.target]);
-                double ratio = 0;
-                if (super) {
-                    const double gt = states[c.target].statWeight, gp = states[c.product].statWeight;
-                    if (gt < 0 || gp <= 0)
-                        throw InputError("Statistical weights of " + states[c.target].raw.name() + " and " +
-                                         states[c.product].raw.name() + " are needed for the superelastic collision of\n" + desc);
-                    ratio = gt / gp;
-                }
-                m.statWeightRatio.push_back(ratio);
-                m.wParameter.push_back(code == EKMC_IONIZATION ? (opb[g] >= 0 ? opb[g] : c.e.threshold) : 0.0);
+                m.gasIndex.push_back(slot);
+                m.threshold.push_back(c.threshold);
+                m.targetMass.push_back(g.mass);
+                m.relDensity.push_back(super ? M.states[prodId].density : tgt.density);
+                m.statWeightRatio.push_back(super ? ratio : 0.0);
+                m.wParameter.push_back(code == EKMC_IONIZATION ? (g.opb != kNonDef ? g.opb : c.threshold) : 0.0);
                 m.scattering.push_back(scatteringCode(c.scattering));
                 m.scatteringParams.push_back(c.params.size() > 0 ? c.params[0] : 0.0);
                 m.scatteringParams.push_back(c.params.size() > 1 ? c.params[1] : 0.0);
                 if (!super) {
-                    m.tableEnergy.insert(m.tableEnergy.end(), en.begin(), en.end());
-                    m.tableSigma.insert(m.tableSigma.end(), sg.begin(), sg.end());
+                    m.tableEnergy.insert(m.tableEnergy.end(), c.integral.e.begin(), c.integral.e.end());
+                    m.tableSigma.insert(m.tableSigma.end(), c.integral.s.begin(), c.integral.s.end());
                 }
                 m.tableOffset.push_back((int32_t)m.tableEnergy.size());
-                m.lxcatType.push_back(c.e.type);
+                m.lxcatType.push_back(c.type);
                 m.collisionId.push_back(cid);
                 m.description.push_back(desc);
             };
             m.collisionFirstProcess.push_back(m.nProcesses());
             push(false);
-            if (c.e.isReverse) push(true);
-            m.collisionIsReverse.push_back(c.e.isReverse ? 1 : 0);
+            if (c.isReverse) push(true);
+            m.collisionIsReverse.push_back(c.isReverse ? 1 : 0);
             m.collisionDescription.push_back(desc);
-            m.collisionTargetDensity.push_back(density[c.target]);
-            m.collisionProductDensity.push_back(c.product >= 0 ? density[c.product] : 0.0);
-            m.collisionEnergy.push_back(en);
-            m.collisionSigmaIntegral.push_back(sg);
-            // isotropic scattering: the momentum-transfer and integral cross sections agree
-            m.collisionSigmaMomTransf.push_back(sg);
-            m.collisionThreshold.push_back(c.e.threshold);
-            m.collisionStatWeightRatio.push_back(c.e.isReverse ? m.statWeightRatio.back() : 0.0);
-            m.collisionType.push_back(c.e.type);
-            m.collisionGas.push_back(gasSlot[g]);
+            m.collisionTargetDensity.push_back(tgt.density);
+            m.collisionProductDensity.push_back(prodId >= 0 ? M.states[prodId].density : 0.0);
+            m.collisionEnergy.push_back(c.integral.e);
+            m.collisionSigmaIntegral.push_back(c.integral.s);
+            m.collisionEnergyMomTransf.push_back(c.momTransf.e);
+            m.collisionSigmaMomTransf.push_back(c.momTransf.s);
+            m.collisionThreshold.push_back(c.threshold);
+            m.collisionStatWeightRatio.push_back(ratio);
+            m.collisionType.push_back(c.type);
+            m.collisionGas.push_back(slot);
         }
     }
     return m;

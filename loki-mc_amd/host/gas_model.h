@@ -56,7 +56,8 @@ struct GasMixture {
     std::vector<std::string> collisionDescription;
     std::vector<double> collisionTargetDensity, collisionProductDensity;
     // EEDF-derived quantities need each collision's cross sections
-    std::vector<std::vector<double>> collisionEnergy, collisionSigmaIntegral, collisionSigmaMomTransf;
+    std::vector<std::vector<double>> collisionEnergy, collisionSigmaIntegral;  // integral cross section
+    std::vector<std::vector<double>> collisionEnergyMomTransf, collisionSigmaMomTransf;  // momentum-transfer
     std::vector<double> collisionThreshold, collisionStatWeightRatio;
     std::vector<std::string> collisionType;
     std::vector<int> collisionGas;
@@ -66,11 +67,11 @@ struct GasMixture {
     ekmc_mixture view() const;
 };
 
-// Build the mixture of `setup` (electronKinetics: LXCatFiles, gasProperties,
-// stateProperties, anisotropicScattering).  Throws InputError for input the
-// reference rejects and for features this reader does not implement
-// (property functions, Effective-only gases, momentum-transfer-only
-// anisotropic collisions).
-GasMixture buildMixture(const SetupFile& setup, const std::string& inputDir);
+// Build the mixture of `setup` (electronKinetics: LXCatFiles(Extra),
+// gasProperties, stateProperties with wildcard states and property functions,
+// effectiveCrossSectionPopulations, anisotropicScattering) at the gas
+// temperature `gasTemperature` [K] (the argument of population functions).
+// Throws InputError for input the reference rejects.
+GasMixture buildMixture(const SetupFile& setup, const std::string& inputDir, double gasTemperature);
 
 }  // namespace lokimc

@@ -142,6 +142,25 @@ void dryRun(const Jobs& jobs, const ekmc_numerics& n, const GasMixture& m) {
                     p ? ", " : "", jsonString(m.description[p]).c_str(), m.type[p], m.isElastic[p], m.isSuperelastic[p],
                     m.gasIndex[p], m.threshold[p], m.relDensity[p], m.statWeightRatio[p], m.wParameter[p], m.scattering[p],
                     m.tableOffset[p + 1] - m.tableOffset[p]);
+    // per collision: densities and both raw cross sections
+    auto array = [](const std::vector<double>& v) {
+        std::string o = "[";
+        char buf[32];
+        for (size_t i = 0; i < v.size(); ++i) {
+            std::snprintf(buf, sizeof buf, "%s%.17g", i ? ", " : "", v[i]);
+            o += buf;
+        }
+        return o + "]";
+    };
+    std::printf("], \"collisions\": [");
+    for (size_t c = 0; c < m.collisionDescription.size(); ++c)
+        std::printf("%s{\"description\": %s, \"type\": %s, \"threshold\": %.17g, \"target_density\": %.17g, "
+                    "\"product_density\": %.17g, \"stat_weight_ratio\": %.17g, \"integral_energy\": %s, \"integral\": %s, "
+                    "\"mom_transf_energy\": %s, \"mom_transf\": %s}",
+                    c ? ", " : "", jsonString(m.collisionDescription[c]).c_str(), jsonString(m.collisionType[c]).c_str(),
+                    m.collisionThreshold[c], m.collisionTargetDensity[c], m.collisionProductDensity[c],
+                    m.collisionStatWeightRatio[c], array(m.collisionEnergy[c]).c_str(), array(m.collisionSigmaIntegral[c]).c_str(),
+                    array(m.collisionEnergyMomTransf[c]).c_str(), array(m.collisionSigmaMomTransf[c]).c_str());
     std::printf("]}\n");
 }
 
@@ -190,7 +209,7 @@ int main(int argc, char** argv) {
         num.seed = seed;
         if (precision == "mixed") num.precision = EKMC_PRECISION_MIXED;
         else if (precision != "fp64") throw InputError("--precision must be fp64 or mixed");
-        const GasMixture mix = buildMixture(setup, inputDir);
+        const GasMixture mix = buildMixture(setup, inputDir, evalNumber(setup.value("workingConditions.gasTemperature")));
         if (dry) {
             dryRun(jobs, num, mix);
             return 0;

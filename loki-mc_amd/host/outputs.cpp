@@ -34,8 +34,9 @@ bool wants(const OutputOptions& o, const char* name) {
 // Cross section of collision c at `energy` (Collision::interpolatedCrossSection,
 // Collision.C:231-289): linear between the table points, zero outside, and
 // zero at or below the threshold of an inelastic collision.
-double sigmaAt(const GasMixture& m, int c, const std::vector<double>& sig, double energy) {
-    const std::vector<double>& e = m.collisionEnergy[c];
+double sigmaAt(const GasMixture& m, int c, bool momTransf, double energy) {
+    const std::vector<double>& e = momTransf ? m.collisionEnergyMomTransf[c] : m.collisionEnergy[c];
+    const std::vector<double>& sig = momTransf ? m.collisionSigmaMomTransf[c] : m.collisionSigmaIntegral[c];
     const bool elasticLike = m.collisionType[c] == "Elastic" || m.collisionType[c] == "Effective";
     if (!elasticLike && !(energy > m.collisionThreshold[c])) return 0.0;
     if (e.empty() || energy < e.front() || energy > e.back()) return 0.0;
@@ -67,15 +68,14 @@ EnergyGrid gridOf(const JobData& j) {
 // Cross section on the nodes; a superelastic one by Klein-Rosseland
 // (Collision.C:291-353).
 std::vector<double> onNodes(const GasMixture& m, int c, const EnergyGrid& g, bool momTransf, bool super) {
-    const std::vector<double>& sig = momTransf ? m.collisionSigmaMomTransf[c] : m.collisionSigmaIntegral[c];
     std::vector<double> out(g.node.size(), 0.0);
     for (size_t i = 0; i < g.node.size(); ++i) {
         const double u = g.node[i];
         if (!super) {
-            out[i] = sigmaAt(m, c, sig, u);
+            out[i] = sigmaAt(m, c, momTransf, u);
         } else if (u != 0) {
             const double thr = m.collisionThreshold[c];
-            out[i] = (1.0 + thr / u) * sigmaAt(m, c, sig, u + thr) * m.collisionStatWeightRatio[c];
+            out[i] = (1.0 + thr / u) * sigmaAt(m, c, momTransf, u + thr) * m.collisionStatWeightRatio[c];
         }
     }
     return out;

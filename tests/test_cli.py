@@ -100,17 +100,26 @@ def test_reversible_collision_gets_a_superelastic_partner(tmp_path):
         "PROCESS: E + X(gnd) -> E + X(gnd), Elastic\nPARAM.: m/M = 1e-4\nCOMMENT: [e + X(gnd) -> e + X(gnd), Elastic]\n"
         "-----\n0 1e-20\n100 1e-20\n-----\n"
         "PROCESS: E + X(gnd) <-> E + X(exc), Excitation\nPARAM.: E = 1.5 eV\nCOMMENT: [e + X(gnd) <-> e + X(exc), Excitation]\n"
-        "-----\n1.5 0\n100 1e-20\n-----\n")
+        "-----\n1.5 0\n100 1e-20\n-----\n"
+        "PROCESS: E + X(exc) -> E + X(exc), Elastic\nPARAM.: m/M = 1e-4\nCOMMENT: [e + X(exc) -> e + X(exc), Elastic]\n"
+        "-----\n0 2e-20\n100 2e-20\n-----\n")
     text = (REID_TEXT.replace("ReidRampGas/ReidRampGas_LXCat.txt", "g.txt").replace("- A = 4.000000*1.660539040e-27", "- X = 1e-26")
             .replace("- A = 1", "- X = 1").replace("- A(gnd) = 1.0", "- X(gnd) = 0.9\n      - X(exc) = 0.1\n    statisticalWeight:\n      - X(gnd) = 1\n      - X(exc) = 3"))
     (inp / "t.in").write_text(text)
     r = _run([str(inp / "t.in"), "--input", str(inp), "--dry-run"], tmp_path)
     assert r.returncode == 0, r.stderr
     p = json.loads(r.stdout)["processes"]
-    assert [x["superelastic"] for x in p] == [0, 0, 1]
+    assert [x["superelastic"] for x in p] == [0, 0, 1, 0]
     assert p[2]["description"] == p[1]["description"] == "e+X(gnd)<->e+X(exc),Excitation"
     assert p[1]["rel_density"] == 0.9 and p[2]["rel_density"] == 0.1     # target / product densities
     assert p[2]["stat_weight_ratio"] == 1.0 / 3.0 and p[2]["points"] == 0
+    assert p[3]["description"] == "e+X(exc)->e+X(exc),Elastic" and p[3]["rel_density"] == 0.1
+    # a reverse product is a target too: without its own Elastic (or an
+    # Effective to derive one from) the setup is rejected (EedfGas.C:120-164)
+    lx = (inp / "g.txt").read_text()
+    (inp / "g.txt").write_text(lx[:lx.index("PROCESS: E + X(exc) -> E + X(exc), Elastic")])
+    r = _run([str(inp / "t.in"), "--input", str(inp), "--dry-run"], tmp_path)
+    assert r.returncode == 1 and "does not have an ''Effective'' collision defined" in r.stderr
 
 
 @pytest.mark.parametrize("change,message", [
