@@ -247,6 +247,15 @@ ekmc_status ekmc_get_eedf(const ekmc_engine* engine, int32_t cond, double* energ
  * per-phase mean energy / flux velocity (BoltzmannMC.C:1725-1741). */
 ekmc_status ekmc_get_eedf_periodic(const ekmc_engine* engine, int32_t cond, double* eedf_periodic,
                                    double* mean_energy_periodic, double* flux_velocity_periodic);
+/* AC only: phase-bin centres [n_integration_phases] (BoltzmannMC.C:525-526)
+ * and the per-phase averages of the sampled swarm quantities after the steady
+ * state (BoltzmannMC.C:1466-1481,1728-1735): mean energy [P], flux and bulk
+ * drift velocity [3P], flux and bulk diffusion coefficients [9P, row-major,
+ * m^2/s] -- what Output::saveMCTemporalInfoPeriodic writes (Output.h:754-782).
+ * Any pointer may be NULL. */
+ekmc_status ekmc_get_swarm_periodic(const ekmc_engine* engine, int32_t cond, double* phases, double* mean_energy,
+                                    double* flux_velocity, double* bulk_velocity, double* flux_diffusion,
+                                    double* bulk_diffusion);
 /* Cylindrically symmetric cases only: EVDF [n_radial * n_axial] row-major with
  * radial/axial cell centres, and the first/second anisotropies per energy cell
  * (BoltzmannMC.C:1583-1605). */

@@ -1445,6 +1445,31 @@ ekmc_status ekmc_get_eedf_periodic(const ekmc_engine* e, int32_t cond, double* p
     return EKMC_OK;
 }
 
+ekmc_status ekmc_get_swarm_periodic(const ekmc_engine* e, int32_t cond, double* phases, double* meanE, double* fluxV,
+                                    double* bulkV, double* fluxD, double* bulkD) {
+    if (!e || cond < 0 || cond >= (int)e->impl->cs.size()) return fail(EKMC_ERR_INVALID_ARGUMENT, "bad argument");
+    const Cond& C = e->impl->cs[cond];
+    if (C.omega == 0) return fail(EKMC_ERR_STATE, "condition has no AC field");
+    const int np = C.nPhases;
+    // bin centres: np points from step/2 to 2 pi - step/2, the last exact
+    const double lo = 0.5 * C.phaseStep, hi = 2.0 * kPi - 0.5 * C.phaseStep;
+    const double step = np > 1 ? (hi - lo) / (double)(np - 1) : 0.0;
+    for (int ph = 0; ph < np; ++ph) {
+        const double n = C.perN[ph];
+        if (phases) phases[ph] = ph == np - 1 ? hi : lo + (double)ph * step;
+        if (meanE) meanE[ph] = C.perE[ph] / n;
+        for (int a = 0; a < 3; ++a) {
+            if (fluxV) fluxV[3 * ph + a] = C.perV[3 * ph + a] / n;
+            if (bulkV) bulkV[3 * ph + a] = C.perBulkV[3 * ph + a] / n;
+        }
+        for (int a = 0; a < 9; ++a) {
+            if (fluxD) fluxD[9 * ph + a] = C.perFlux[9 * ph + a] / n;
+            if (bulkD) bulkD[9 * ph + a] = C.perBulkD[9 * ph + a] / n;
+        }
+    }
+    return EKMC_OK;
+}
+
 ekmc_status ekmc_get_evdf(const ekmc_engine* e, int32_t cond, double* radial, double* axial, double* evdf, double* efadf,
                           double* esadf) {
     if (!e || cond < 0 || cond >= (int)e->impl->cs.size()) return fail(EKMC_ERR_INVALID_ARGUMENT, "bad argument");

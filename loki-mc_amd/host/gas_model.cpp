@@ -1125,6 +1125,49 @@ GasMixture buildMixture(const SetupFile& setup, const std::string& inputDir, dou
             m.collisionStatWeightRatio.push_back(ratio);
             m.collisionType.push_back(c.type);
             m.collisionGas.push_back(slot);
+            m.collisionKind.push_back(0);
+            m.collisionRefId.push_back(k);
+        }
+    }
+    // collisions listed in the rate-coefficient files only (BoltzmannMC.C:
+    // 2361-2382): the Effective ones of every gas, then the extra ones
+    for (int kind = 1; kind <= 2; ++kind) {
+        for (size_t gi = 0; gi < M.gases.size(); ++gi) {
+            int slot = -1;
+            for (size_t q = 0; q < m.gasNames.size(); ++q)
+                if (m.gasNames[q] == M.gases[gi].name) slot = (int)q;
+            for (size_t k = 0; k < M.colls.size(); ++k) {
+                const Collision& c = M.colls[k];
+                if (M.states[c.target].gas != (int)gi) continue;
+                if (kind == 1 ? (c.extra || c.type != "Effective") : !c.extra) continue;
+                const XSec& x = c.integral.defined() ? c.integral : c.momTransf;
+                const XSec& mt = c.momTransf.defined() ? c.momTransf : c.integral;
+                double ratio = 0;
+                if (c.isReverse) {
+                    const State& t = M.states[c.target];
+                    const State& pr = M.states[c.products[0]];
+                    if (t.statWeight == kNonDef || pr.statWeight == kNonDef)
+                        throw InputError("Unable to find '" + (t.statWeight == kNonDef ? t : pr).raw.name() +
+                                         "' statistical weight for the evaluation of superelastic rate coefficient of " +
+                                         M.description(c) + "\n");
+                    ratio = t.statWeight / pr.statWeight;
+                }
+                m.collisionFirstProcess.push_back(-1);
+                m.collisionIsReverse.push_back(c.isReverse ? 1 : 0);
+                m.collisionDescription.push_back(M.description(c));
+                m.collisionTargetDensity.push_back(M.states[c.target].density);
+                m.collisionProductDensity.push_back(c.products.empty() ? 0.0 : M.states[c.products[0]].density);
+                m.collisionEnergy.push_back(x.e);
+                m.collisionSigmaIntegral.push_back(x.s);
+                m.collisionEnergyMomTransf.push_back(mt.e);
+                m.collisionSigmaMomTransf.push_back(mt.s);
+                m.collisionThreshold.push_back(c.threshold);
+                m.collisionStatWeightRatio.push_back(ratio);
+                m.collisionType.push_back(c.type);
+                m.collisionGas.push_back(slot);
+                m.collisionKind.push_back(kind);
+                m.collisionRefId.push_back((int)k);
+            }
         }
     }
     return m;

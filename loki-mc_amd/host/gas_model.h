@@ -60,7 +60,12 @@ struct GasMixture {
     std::vector<std::vector<double>> collisionEnergyMomTransf, collisionSigmaMomTransf;  // momentum-transfer
     std::vector<double> collisionThreshold, collisionStatWeightRatio;
     std::vector<std::string> collisionType;
-    std::vector<int> collisionGas;
+    std::vector<int> collisionGas;         // slot in gasNames, -1 for a gas without flights
+    // 0: a collision of the flights (first the process order's collisions);
+    // 1: an Effective collision, 2: an extra one (LXCatFilesExtra), both
+    // listed in the rate-coefficient files only, after the kind-0 ones
+    std::vector<int> collisionKind;
+    std::vector<int> collisionRefId;       // the reference's Collision::ID (creation order), printed + 1
 
     int nProcesses() const { return (int)type.size(); }
     // View for the C ABI (pointers into this object).

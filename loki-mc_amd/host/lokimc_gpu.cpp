@@ -154,10 +154,11 @@ void dryRun(const Jobs& jobs, const ekmc_numerics& n, const GasMixture& m) {
     };
     std::printf("], \"collisions\": [");
     for (size_t c = 0; c < m.collisionDescription.size(); ++c)
-        std::printf("%s{\"description\": %s, \"type\": %s, \"threshold\": %.17g, \"target_density\": %.17g, "
+        std::printf("%s{\"description\": %s, \"kind\": %d, \"ref_id\": %d, \"type\": %s, \"threshold\": %.17g, \"target_density\": %.17g, "
                     "\"product_density\": %.17g, \"stat_weight_ratio\": %.17g, \"integral_energy\": %s, \"integral\": %s, "
                     "\"mom_transf_energy\": %s, \"mom_transf\": %s}",
-                    c ? ", " : "", jsonString(m.collisionDescription[c]).c_str(), jsonString(m.collisionType[c]).c_str(),
+                    c ? ", " : "", jsonString(m.collisionDescription[c]).c_str(), m.collisionKind[c], m.collisionRefId[c],
+                    jsonString(m.collisionType[c]).c_str(),
                     m.collisionThreshold[c], m.collisionTargetDensity[c], m.collisionProductDensity[c],
                     m.collisionStatWeightRatio[c], array(m.collisionEnergy[c]).c_str(), array(m.collisionSigmaIntegral[c]).c_str(),
                     array(m.collisionEnergyMomTransf[c]).c_str(), array(m.collisionSigmaMomTransf[c]).c_str());

@@ -12,6 +12,7 @@ namespace {
 
 constexpr double kCharge = 1.6021766208e-19;  // Constant.h:6-23
 constexpr double kMass = 9.10938356e-31;
+constexpr double kPi = 3.14159265358979323846;  // M_PI
 
 void check(ekmc_status s, const char* what) {
     if (s != EKMC_OK) throw std::runtime_error(std::string(what) + ": " + ekmc_last_error());
@@ -129,12 +130,30 @@ JobData collectJob(const ekmc_engine* eng, int cond, const ekmc_condition& c, co
     j.positionCovariance.resize(9 * (size_t)n);
     int32_t nc = n;
     check(ekmc_get_position_covariance(eng, cond, &nc, j.positionCovariance.data()), "ekmc_get_position_covariance");
+    if (c.excitation_frequency != 0) {
+        const int np = num.n_integration_phases, ncell = num.n_energy_cells;
+        j.nPhases = np;
+        j.phases.resize(np);
+        j.eedfPeriodic.resize((size_t)np * ncell);
+        j.meanEnergyPeriodic.resize(np);
+        j.fluxVelocityPeriodic.resize(3 * (size_t)np);
+        j.bulkVelocityPeriodic.resize(3 * (size_t)np);
+        j.fluxDiffusionPeriodic.resize(9 * (size_t)np);
+        j.bulkDiffusionPeriodic.resize(9 * (size_t)np);
+        check(ekmc_get_eedf_periodic(eng, cond, j.eedfPeriodic.data(), nullptr, nullptr), "ekmc_get_eedf_periodic");
+        check(ekmc_get_swarm_periodic(eng, cond, j.phases.data(), j.meanEnergyPeriodic.data(), j.fluxVelocityPeriodic.data(),
+                                      j.bulkVelocityPeriodic.data(), j.fluxDiffusionPeriodic.data(),
+                                      j.bulkDiffusionPeriodic.data()),
+              "ekmc_get_swarm_periodic");
+    }
     return j;
 }
 
-void eedfRateCoefficients(const JobData& j, const GasMixture& m, std::vector<double>& ine, std::vector<double>& sup) {
+void eedfRateCoefficients(const JobData& j, const GasMixture& m, std::vector<double>& ine, std::vector<double>& sup,
+                          const double* eedf) {
     // Collision::evaluateRateCoeff (Collision.C:355-397)
     const EnergyGrid g = gridOf(j);
+    if (!eedf) eedf = j.eedf.data();
     const double factor = std::sqrt(2.0 * kCharge / kMass);
     const int nc = (int)m.collisionDescription.size();
     ine.assign(nc, 0.0);
@@ -151,8 +170,8 @@ void eedfRateCoefficients(const JobData& j, const GasMixture& m, std::vector<dou
         double si = 0, ss = 0;
         for (int k = 0; k < part; ++k) {
             const double aux = 0.5 * (sig[lmin + k] + sig[lmin + k + 1]) * g.cell[lmin + k];
-            si += aux * j.eedf[lmin + k];
-            ss += aux * j.eedf[k];
+            si += aux * eedf[lmin + k];
+            ss += aux * eedf[k];
         }
         ine[c] = factor * si * g.step;
         if (m.collisionIsReverse[c]) sup[c] = factor * m.collisionStatWeightRatio[c] * ss * g.step;
@@ -172,6 +191,7 @@ SwarmParams swarmParameters(const JobData& j, const GasMixture& m) {
     eedfRateCoefficients(j, m, ine, sup);
     double ion = 0, att = 0;
     for (size_t c = 0; c < ine.size(); ++c) {
+        if (m.collisionKind[c] != 0) continue;
         if (m.collisionType[c] == "Ionization") ion += m.collisionTargetDensity[c] * ine[c];
         else if (m.collisionType[c] == "Attachment") att += m.collisionTargetDensity[c] * ine[c];
     }
@@ -216,6 +236,7 @@ SwarmParams swarmParameters(const JobData& j, const GasMixture& m) {
     const double factor = std::sqrt(2.0 * kCharge / kMass) / 3.0;
     std::vector<double> mt(Nn, 0.0), er(Nn, 0.0), erc(Nn, 0.0);
     for (size_t c = 0; c < m.collisionDescription.size(); ++c) {
+        if (m.collisionKind[c] != 0) continue;  // Effective and extra collisions are not part of the gas's set
         const double dens = m.collisionTargetDensity[c];
         const std::vector<double> smt = onNodes(m, (int)c, g, true, false), sint = onNodes(m, (int)c, g, false, false);
         const double ratio = kMass / m.gasMass[m.collisionGas[c]];
@@ -341,6 +362,15 @@ void writeEedf(const std::string& dir, const JobData& j) {
         std::fprintf(out.f, "Energy(eV)           EEDF(eV^-(3/2))      First Anisotropy     Second Anisotropy    \n");
         for (size_t i = 0; i < j.eedf.size(); ++i)
             std::fprintf(out.f, "%-20.14e %-20.14e %-20.14e %-20.14e\n", j.energyCells[i], j.eedf[i], j.efadf[i], j.esadf[i]);
+    }
+    if (j.cond.excitation_frequency == 0) return;
+    // one EEDF per phase of the period (Output.h:207-220)
+    File per(dir + "/eedf_periodic.txt");
+    const size_t nc = j.eedf.size();
+    for (int ph = 0; ph < j.nPhases; ++ph) {
+        std::fprintf(per.f, "%-20.14e ", j.phases[ph]);
+        for (size_t i = 0; i < nc; ++i) std::fprintf(per.f, "%-20.14e ", j.eedfPeriodic[(size_t)ph * nc + i]);
+        std::fprintf(per.f, "\n");
     }
 }
 
@@ -509,6 +539,57 @@ void writePower(const std::string& dir, const Power& pw) {
     }
 }
 
+// Rows of the rate-coefficient files (Output.h:419-538): the collisions of
+// the flights, then the Effective ones, then (in their own section) the extra
+// ones, each with the reference's collision ID.
+void rateRow(FILE* f, const GasMixture& m, int c, double ine, double sup) {
+    if (!m.collisionIsReverse[c])
+        std::fprintf(f, "%4d %20.14e (N/A)                %s\n", m.collisionRefId[c] + 1, ine, m.collisionDescription[c].c_str());
+    else
+        std::fprintf(f, "%4d %20.14e %20.14e %s\n", m.collisionRefId[c] + 1, ine, sup, m.collisionDescription[c].c_str());
+}
+
+bool hasExtra(const GasMixture& m) {
+    for (int k : m.collisionKind)
+        if (k == 2) return true;
+    return false;
+}
+
+// The "# ID Description" block and the column header of a periodic or
+// look-up rate-coefficient file (Output.h:478-507, 620-640).
+std::string rateColumns(FILE* f, const GasMixture& m, std::string header) {
+    std::fprintf(f, "%s\n# %-76s #\n", std::string(80, '#').c_str(), "ID   Description");
+    auto column = [&](int c) {
+        const int id = m.collisionRefId[c] + 1;
+        std::fprintf(f, "# %-4d %-71s #\n", id, m.collisionDescription[c].c_str());
+        std::string a = "R" + std::to_string(id) + "_ine(m3/s)";
+        header += a + std::string(a.size() < 22 ? 22 - a.size() : 1, ' ');
+        if (m.collisionIsReverse[c]) {
+            a = "R" + std::to_string(id) + "_sup(m3/s)";
+            header += a + std::string(a.size() < 22 ? 22 - a.size() : 1, ' ');
+        }
+    };
+    const int nc = (int)m.collisionDescription.size();
+    for (int c = 0; c < nc; ++c)
+        if (m.collisionKind[c] != 2) column(c);
+    std::fprintf(f, "#%s#\n# %-76s #\n#%s#\n# %-76s #\n", std::string(78, ' ').c_str(), "*** Extra rate coefficients ***",
+                 std::string(78, '#').c_str(), "ID   Description");
+    for (int c = 0; c < nc; ++c)
+        if (m.collisionKind[c] == 2) column(c);
+    std::fprintf(f, "%s\n\n%s\n", std::string(80, '#').c_str(), header.c_str());
+    return header;
+}
+
+void rateValues(FILE* f, const GasMixture& m, const std::vector<double>& ine, const std::vector<double>& sup) {
+    const int nc = (int)m.collisionDescription.size();
+    for (int pass = 0; pass < 2; ++pass)
+        for (int c = 0; c < nc; ++c) {
+            if ((m.collisionKind[c] == 2) != (pass == 1)) continue;
+            std::fprintf(f, "%-21.14e ", ine[c]);
+            if (m.collisionIsReverse[c]) std::fprintf(f, "%-21.14e ", sup[c]);
+        }
+}
+
 void writeRates(const std::string& dir, const JobData& j, const GasMixture& m) {
     std::vector<double> ine, sup;
     eedfRateCoefficients(j, m, ine, sup);
@@ -516,21 +597,38 @@ void writeRates(const std::string& dir, const JobData& j, const GasMixture& m) {
     {
         File out(dir + "/rateCoefficients.txt");
         std::fprintf(out.f, " ID  Ine.R.Coeff.(m3/s)   Sup.R.Coeff.(m3/s)   Description\n");
-        for (int c = 0; c < nc; ++c) {
-            if (!m.collisionIsReverse[c])
-                std::fprintf(out.f, "%4d %20.14e (N/A)                %s\n", c + 1, ine[c], m.collisionDescription[c].c_str());
-            else
-                std::fprintf(out.f, "%4d %20.14e %20.14e %s\n", c + 1, ine[c], sup[c], m.collisionDescription[c].c_str());
+        for (int c = 0; c < nc; ++c)
+            if (m.collisionKind[c] != 2) rateRow(out.f, m, c, ine[c], sup[c]);
+        if (hasExtra(m)) {
+            std::fprintf(out.f, "\n%s\n* Extra Rate Coefficients *\n%s\n\n", std::string(27, '*').c_str(), std::string(27, '*').c_str());
+            for (int c = 0; c < nc; ++c)
+                if (m.collisionKind[c] == 2) rateRow(out.f, m, c, ine[c], sup[c]);
         }
     }
-    File out(dir + "/rateCoefficientsMC.txt");
-    std::fprintf(out.f, " ID  Ine.R.Coeff.(m3/s)   Sup.R.Coeff.(m3/s)   Description\n");
-    for (int c = 0; c < nc; ++c) {
-        const int p = m.collisionFirstProcess[c];
-        if (!m.collisionIsReverse[c])
-            std::fprintf(out.f, "%4d %20.14e (N/A)                %s\n", c + 1, j.rateMC[p], m.collisionDescription[c].c_str());
-        else
-            std::fprintf(out.f, "%4d %20.14e %20.14e %s\n", c + 1, j.rateMC[p], j.rateMC[p + 1], m.collisionDescription[c].c_str());
+    {
+        File out(dir + "/rateCoefficientsMC.txt");
+        std::fprintf(out.f, " ID  Ine.R.Coeff.(m3/s)   Sup.R.Coeff.(m3/s)   Description\n");
+        for (int c = 0; c < nc; ++c) {
+            if (m.collisionKind[c] != 0) continue;
+            const int p = m.collisionFirstProcess[c];
+            rateRow(out.f, m, c, j.rateMC[p], m.collisionIsReverse[c] ? j.rateMC[p + 1] : 0.0);
+        }
+    }
+    const double omega = j.cond.excitation_frequency * 2.0 * kPi;
+    if (omega == 0) return;
+    // rate coefficients of the EEDF of each phase (Output.h:468-535)
+    File out(dir + "/rateCoefficients_periodic.txt");
+    std::string header;
+    for (const char* v : {"Phase(rad)", "Phase(s)", "E/N(Td)"}) header += std::string(v) + std::string(22 - std::string(v).size(), ' ');
+    rateColumns(out.f, m, header);
+    const double EN = j.cond.reduced_elec_field;
+    const size_t ncell = j.eedf.size();
+    for (int ph = 0; ph < j.nPhases; ++ph) {
+        std::vector<double> pi, ps;
+        eedfRateCoefficients(j, m, pi, ps, j.eedfPeriodic.data() + (size_t)ph * ncell);
+        std::fprintf(out.f, "%-21.14e %-21.14e %-21.14e ", j.phases[ph], j.phases[ph] / omega, std::sqrt(2.0) * EN * std::cos(j.phases[ph]));
+        rateValues(out.f, m, pi, ps);
+        std::fprintf(out.f, "\n");
     }
 }
 
@@ -587,6 +685,32 @@ void writeDetails(const std::string& dir, const JobData& j) {
 
 }  // namespace
 
+// Output.h:754-782
+void writeTemporalPeriodic(const std::string& dir, const JobData& j) {
+    const double omega = j.cond.excitation_frequency * 2.0 * kPi;
+    if (omega == 0) return;
+    File out(dir + "/MCTemporalInfo_periodic.txt");
+    const char* vars[] = {"Phase(rad)", "Phase(s)", "E/N(Td)", "MeanEnergy(eV)", "FluxV_x(m/s)", "FluxV_y(m/s)", "FluxV_z(m/s)",
+                          "BulkV_x(m/s)", "BulkV_y(m/s)", "BulkV_z(m/s)", "FluxND_xx(1/(ms))", "FluxND_xy(1/(ms))",
+                          "FluxND_xz(1/(ms))", "FluxND_yx(1/(ms))", "FluxND_yy(1/(ms))", "FluxND_yz(1/(ms))",
+                          "FluxND_zx(1/(ms))", "FluxND_zy(1/(ms))", "FluxND_zz(1/(ms))", "BulkND_xx(1/(ms))",
+                          "BulkND_xy(1/(ms))", "BulkND_xz(1/(ms))", "BulkND_yx(1/(ms))", "BulkND_yy(1/(ms))",
+                          "BulkND_yz(1/(ms))", "BulkND_zx(1/(ms))", "BulkND_zy(1/(ms))", "BulkND_zz(1/(ms))"};
+    std::string header;
+    for (const char* v : vars) header += std::string(v) + std::string(20 - std::string(v).size(), ' ');
+    std::fprintf(out.f, "%s\n", header.c_str());
+    const double EN = j.cond.reduced_elec_field, N = j.cond.gas_density;
+    for (int ph = 0; ph < j.nPhases; ++ph) {
+        std::vector<double> row = {j.phases[ph], j.phases[ph] / omega, std::sqrt(2.0) * EN * std::cos(j.phases[ph]),
+                                   j.meanEnergyPeriodic[ph]};
+        for (int a = 0; a < 3; ++a) row.push_back(j.fluxVelocityPeriodic[3 * ph + a]);
+        for (int a = 0; a < 3; ++a) row.push_back(j.bulkVelocityPeriodic[3 * ph + a]);
+        for (int a = 0; a < 9; ++a) row.push_back(N * j.fluxDiffusionPeriodic[9 * ph + a]);
+        for (int a = 0; a < 9; ++a) row.push_back(N * j.bulkDiffusionPeriodic[9 * ph + a]);
+        for (size_t k = 0; k < row.size(); ++k) std::fprintf(out.f, k + 1 < row.size() ? "%-19.10e " : "%-19.10e\n", row[k]);
+    }
+}
+
 void writeJobFiles(const std::string& dir, const JobData& j, const GasMixture& m, const OutputOptions& o) {
     if (wants(o, "eedf")) writeEedf(dir, j);
     if (wants(o, "evdf") && j.cylindrical) writeEvdf(dir, j);
@@ -594,11 +718,13 @@ void writeJobFiles(const std::string& dir, const JobData& j, const GasMixture& m
     if (wants(o, "swarmParameters")) writeSwarm(dir, j, swarmParameters(j, m));
     if (wants(o, "rateCoefficients")) writeRates(dir, j, m);
     if (wants(o, "MCTemporalInfo")) writeTemporal(dir, j);
+    if (wants(o, "MCTemporalInfo_periodic")) writeTemporalPeriodic(dir, j);
     if (wants(o, "MCSimDetails")) writeDetails(dir, j);
 }
 
 void writeLookUpTables(const std::string& dir, const std::vector<JobData>& jobs, const GasMixture& m,
                        const OutputOptions& o) {
+    // Output.h:540-728 (boltzmannMC branch), written once all jobs are done
     if (!wants(o, "lookUpTable") || jobs.size() < 2) return;
     std::string varCond = "RedElecField(Td)      ";
     if (o.variableCondition == "reducedMagField") varCond = "RedMagField(Hx)       ";
@@ -610,9 +736,10 @@ void writeLookUpTables(const std::string& dir, const std::vector<JobData>& jobs,
         if (o.variableCondition == "excitationFrequency") return j.cond.excitation_frequency;
         return j.cond.reduced_elec_field;
     };
-    auto pad = [](const std::string& s) { return s + std::string(s.size() < 22 ? 22 - s.size() : 1, ' '); };
+    auto pad = [](const std::string& s, size_t w) { return s + std::string(s.size() < w ? w - s.size() : 1, ' '); };
+    File swarm(dir + "/lookUpTableSwarm.txt"), power(dir + "/lookUpTablePower.txt"), rate(dir + "/lookUpTableRateCoeff.txt"),
+        eedf(dir + "/lookUpTableEedf.txt"), grid(dir + "/lookUpTableEGrid.txt");
     {
-        File out(dir + "/lookUpTableSwarm.txt");
         const char* vars[] = {"FluxND_xx(1/(ms))", "FluxND_xy(1/(ms))", "FluxND_xz(1/(ms))", "FluxND_yx(1/(ms))", "FluxND_yy(1/(ms))",
                               "FluxND_yz(1/(ms))", "FluxND_zx(1/(ms))", "FluxND_zy(1/(ms))", "FluxND_zz(1/(ms))", "FluxV_x(m/s)",
                               "FluxV_y(m/s)", "FluxV_z(m/s)", "BulkND_xx(1/(ms))", "BulkND_xy(1/(ms))", "BulkND_xz(1/(ms))",
@@ -622,64 +749,60 @@ void writeLookUpTables(const std::string& dir, const std::vector<JobData>& jobs,
                               "MomTransfFreq(1/s)", "EnergyRelFreq(1/s)", "RedDiffE_eedf(eV/(ms))", "RedMobE_eedf(eV/(msV))",
                               "RedDiff_eedf(1/(ms))", "RedMob_DC_eedf(1/(msV))", "CharacEnergy_eedf(eV)", "trialCollFreq(1/s)"};
         std::string h = varCond;
-        for (const char* v : vars) h += pad(v);
-        std::fprintf(out.f, "%s\n", h.c_str());
-        for (const JobData& j : jobs) {
-            const SwarmParams s = swarmParameters(j, m);
-            const ekmc_results& r = j.res;
-            const double N = j.cond.gas_density;
-            std::vector<double> row;
-            row.push_back(var(j));
-            for (int k = 0; k < 9; ++k) row.push_back(N * r.rotated_flux_diffusion[k]);
-            for (int k = 0; k < 3; ++k) row.push_back(r.rotated_flux_drift_velocity[k]);
-            for (int k = 0; k < 9; ++k) row.push_back(N * r.rotated_bulk_diffusion[k]);
-            for (int k = 0; k < 3; ++k) row.push_back(r.rotated_bulk_drift_velocity[k]);
-            for (const char* k : {"effSSTAverageVelocity", "effSSTRedTownsendCoeff", "effSSTRedAttCoeff", "meanEnergy", "Te",
-                                  "totalIonRateCoeff", "totalAttRateCoeff", "momTransfFreq", "energyRelaxFreq",
-                                  "redDiffCoeffEnergy_eedf", "redMobCoeffEnergy_eedf", "redDiffCoeff_eedf",
-                                  "redMobCoeff_DC_eedf", "characEnergy_eedf"})
-                row.push_back(s.get(k));
-            row.push_back(r.trial_collision_frequency);
-            for (double v : row) std::fprintf(out.f, "%-+21.14e ", v);
-            std::fprintf(out.f, "\n");
-        }
+        for (const char* v : vars) h += pad(v, 24);
+        std::fprintf(swarm.f, "%s\n", h.c_str());
+        const char* pvars[] = {"PowerField(eVm3/s)", "PwrElaGain(eVm3/s)", "PwrElaLoss(eVm3/s)", "PwrElaNet(eVm3/s)",
+                               "PwrEleGain(eVm3/s)", "PwrEleLoss(eVm3/s)", "PwrEleNet(eVm3/s)", "PwrVibGain(eVm3/s)",
+                               "PwrVibLoss(eVm3/s)", "PwrVibNet(eVm3/s)", "PwrRotGain(eVm3/s)", "PwrRotLoss(eVm3/s)",
+                               "PwrRotNet(eVm3/s)", "PwrIon(eVm3/s)", "PwrAtt(eVm3/s)", "PwrGrowth(eVm3/s)",
+                               "PwrBalance(eVm3/s)", "RelPwrBalance"};
+        h = varCond;
+        for (const char* v : pvars) h += pad(v, 22);
+        std::fprintf(power.f, "%s\n", h.c_str());
+        rateColumns(rate.f, m, varCond);
     }
-    {
-        File out(dir + "/lookUpTablePower.txt");
-        const char* keys[] = {"field", "elasticNet", "carNet", "excitationIne", "excitationSup", "vibrationalIne",
-                              "vibrationalSup", "rotationalIne", "rotationalSup", "ionizationIne", "attachmentIne",
-                              "eDensGrowth", "balance", "relativeBalance"};
-        const char* heads[] = {"PowerField(eVm3/s)", "PwrElasticNet(eVm3/s)", "PwrCARNet(eVm3/s)", "PwrExcIne(eVm3/s)",
-                               "PwrExcSup(eVm3/s)", "PwrVibIne(eVm3/s)", "PwrVibSup(eVm3/s)", "PwrRotIne(eVm3/s)",
-                               "PwrRotSup(eVm3/s)", "PwrIon(eVm3/s)", "PwrAtt(eVm3/s)", "PwrGrowth(eVm3/s)",
-                               "PowerBalance(eVm3/s)", "RelPowerBalance"};
-        std::string h = varCond;
-        for (const char* v : heads) h += pad(v);
-        std::fprintf(out.f, "%s\n", h.c_str());
-        for (const JobData& j : jobs) {
-            Power p = powerOf(j, m);
-            std::fprintf(out.f, "%-+21.14e ", var(j));
-            for (const char* k : keys) std::fprintf(out.f, "%-+21.14e ", p.total[k]);
-            std::fprintf(out.f, "\n");
-        }
-    }
-    File out(dir + "/lookUpTableRateCoeff.txt");
-    std::string h = varCond;
-    const int nc = (int)m.collisionDescription.size();
-    for (int c = 0; c < nc; ++c) {
-        h += pad("R" + std::to_string(c + 1) + "_ine(m3/s)");
-        if (m.collisionIsReverse[c]) h += pad("R" + std::to_string(c + 1) + "_sup(m3/s)");
-    }
-    std::fprintf(out.f, "%s\n", h.c_str());
     for (const JobData& j : jobs) {
+        const double cv = var(j);
+        const SwarmParams s = swarmParameters(j, m);
+        const ekmc_results& r = j.res;
+        const double N = j.cond.gas_density;
+        std::vector<double> row;
+        for (int k = 0; k < 9; ++k) row.push_back(r.flux_diffusion[k] * N);
+        for (int k = 0; k < 3; ++k) row.push_back(r.flux_drift_velocity[k]);
+        for (int k = 0; k < 9; ++k) row.push_back(r.bulk_diffusion[k] * N);
+        for (int k = 0; k < 3; ++k) row.push_back(r.bulk_drift_velocity[k]);
+        for (const char* k : {"effSSTAverageVelocity", "effSSTRedTownsendCoeff", "effSSTRedAttCoeff", "meanEnergy", "Te",
+                              "totalIonRateCoeff", "totalAttRateCoeff", "momTransfFreq", "energyRelaxFreq",
+                              "redDiffCoeffEnergy_eedf", "redMobCoeffEnergy_eedf", "redDiffCoeff_eedf",
+                              "redMobCoeff_DC_eedf", "characEnergy_eedf"})
+            row.push_back(s.get(k));
+        row.push_back(r.trial_collision_frequency);
+        std::fprintf(swarm.f, "%-21.14e ", cv);
+        for (double v : row) std::fprintf(swarm.f, "%-23.14e ", v);
+        std::fprintf(swarm.f, "\n");
+
+        Power p = powerOf(j, m);
+        std::fprintf(power.f, "%-21.14e ", cv);
+        for (const char* k : {"field", "elasticGain", "elasticLoss", "elasticNet", "excitationSup", "excitationIne",
+                              "excitationNet", "vibrationalSup", "vibrationalIne", "vibrationalNet", "rotationalSup",
+                              "rotationalIne", "rotationalNet", "ionizationIne", "attachmentIne", "eDensGrowth", "balance"})
+            std::fprintf(power.f, "%-21.14e ", p.total[k]);
+        std::fprintf(power.f, "%19.14e%%\n", p.total["relativeBalance"] * 100.0);
+
         std::vector<double> ine, sup;
         eedfRateCoefficients(j, m, ine, sup);
-        std::fprintf(out.f, "%-+21.14e ", var(j));
-        for (int c = 0; c < nc; ++c) {
-            std::fprintf(out.f, "%-+21.14e ", ine[c]);
-            if (m.collisionIsReverse[c]) std::fprintf(out.f, "%-+21.14e ", sup[c]);
+        std::fprintf(rate.f, "%-21.14e ", cv);
+        rateValues(rate.f, m, ine, sup);
+        std::fprintf(rate.f, "\n");
+
+        std::fprintf(eedf.f, "%-20.14e ", cv);
+        std::fprintf(grid.f, "%-20.14e ", cv);
+        for (size_t i = 0; i < j.energyCells.size(); ++i) {
+            std::fprintf(eedf.f, "%-20.14e ", j.eedf[i]);
+            std::fprintf(grid.f, "%-20.14e ", j.energyCells[i]);
         }
-        std::fprintf(out.f, "\n");
+        std::fprintf(eedf.f, "\n");
+        std::fprintf(grid.f, "\n");
     }
 }
 

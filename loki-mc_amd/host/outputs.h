@@ -24,6 +24,11 @@ struct JobData {
     std::vector<double> times, meanEnergy, meanPosition, meanVelocity, fluxDiffusion, bulkVelocity, bulkDiffusion,
         positionCovariance;
     double nElectrons = 0;
+    // AC conditions: phase-bin centres, EEDF per phase [phase][cell], and the
+    // per-phase averages of the sampled swarm quantities
+    int nPhases = 0;
+    std::vector<double> phases, eedfPeriodic, meanEnergyPeriodic, fluxVelocityPeriodic, bulkVelocityPeriodic,
+        fluxDiffusionPeriodic, bulkDiffusionPeriodic;
 };
 
 // Pull a condition's results out of a finished engine.
@@ -38,8 +43,10 @@ struct SwarmParams {
 SwarmParams swarmParameters(const JobData& j, const GasMixture& m);
 
 // EEDF-convolution rate coefficients per collision (inelastic, superelastic
-// or NaN when the collision is not reversible).
-void eedfRateCoefficients(const JobData& j, const GasMixture& m, std::vector<double>& ine, std::vector<double>& sup);
+// or NaN when the collision is not reversible), for the job's EEDF or, with
+// `eedf`, for that EEDF on the job's cells.
+void eedfRateCoefficients(const JobData& j, const GasMixture& m, std::vector<double>& ine, std::vector<double>& sup,
+                          const double* eedf = nullptr);
 
 // Output files requested by `dataFiles` (output.dataFiles) into `folder`
 // (the job's sub-folder when there are several jobs).

@@ -374,6 +374,7 @@ def lib():
         L.ekmc_get_process_data.argtypes = [C.c_void_p, I32, pD, pU64, pD, pD]
         L.ekmc_get_eedf.argtypes = [C.c_void_p, I32, pD, pD]
         L.ekmc_get_eedf_periodic.argtypes = [C.c_void_p, I32, pD, pD, pD]
+        L.ekmc_get_swarm_periodic.argtypes = [C.c_void_p, I32, pD, pD, pD, pD, pD, pD]
         L.ekmc_get_evdf.argtypes = [C.c_void_p, I32, pD, pD, pD, pD, pD]
         L.ekmc_get_series.argtypes = [C.c_void_p, I32, pI32, pD, pD, pD, pD, pD, pD, pD]
         L.ekmc_get_electrons.argtypes = [C.c_void_p, I32, pD, pD, pD, pD]

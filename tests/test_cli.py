@@ -170,4 +170,59 @@ def test_reid_ramp_run_writes_the_reference_output_files(tmp_path):
         pb = (sub / "powerBalance.txt").read_text()
         rel_balance = float(re.search(r"Relative Power Balance = (\S+)%", pb).group(1))
         assert rel_balance < 1.0
-    assert (out / "lookUpTableSwarm.txt").exists()
+    # look-up tables (Output.h:540-728): one row per job
+    swarm = (out / "lookUpTableSwarm.txt").read_text().splitlines()
+    assert swarm[0].startswith("RedElecField(Td)      FluxND_xx(1/(ms))       ") and len(swarm) == 1 + len(REID)
+    assert all(len(l.split()) == 40 for l in swarm[1:])
+    power = (out / "lookUpTablePower.txt").read_text().splitlines()
+    assert power[0].startswith("RedElecField(Td)      PowerField(eVm3/s)    PwrElaGain(eVm3/s)") and len(power) == 4
+    assert all(l.endswith("%") and len(l.split()) == 19 for l in power[1:])
+    rates = (out / "lookUpTableRateCoeff.txt").read_text().splitlines()
+    assert rates[0] == "#" * 80 and rates[1].startswith("# ID   Description")
+    assert "# 1    e+A(gnd)->e+A(gnd),Elastic" in rates[2]
+    assert any("*** Extra rate coefficients ***" in l for l in rates)
+    eedf_rows = (out / "lookUpTableEedf.txt").read_text().splitlines()
+    grid_rows = (out / "lookUpTableEGrid.txt").read_text().splitlines()
+    assert len(eedf_rows) == len(grid_rows) == len(REID)
+    for row, grow, en in zip(eedf_rows, grid_rows, sorted(REID)):
+        e = np.loadtxt(out / ("reducedElecField_%g" % en) / "eedf.txt", skiprows=1)
+        vals, gvals = [float(x) for x in row.split()], [float(x) for x in grow.split()]
+        assert vals[0] == gvals[0] == en and len(vals) == 1 + len(e)
+        np.testing.assert_allclose(vals[1:], e[:, 1], rtol=1e-13)
+        np.testing.assert_allclose(gvals[1:], e[:, 0], rtol=1e-13)
+
+
+@pytest.mark.gpu
+def test_ac_field_run_writes_the_periodic_files(tmp_path):
+    inp = _input_tree(tmp_path)
+    text = (REID_TEXT.replace("reducedElecField: [1,12,24]", "reducedElecField: 12")
+            .replace("excitationFrequency: 0", "excitationFrequency: 2e8")
+            .replace("nElectrons: 1E5", "nElectrons: 2E4\n    nIntegrationPhases: 8")
+            .replace("nIntegrationPoints: 1E4", "nIntegrationPoints: 4E3"))
+    (inp / "ac.in").write_text(text)
+    r = _run([str(inp / "ac.in"), "--input", str(inp), "--output", str(tmp_path / "Output"), "--seed", "3"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    out = tmp_path / "Output" / "ReidRamp_DC"
+    phases = (np.arange(8) + 0.5) * 2 * np.pi / 8
+    per = np.loadtxt(out / "eedf_periodic.txt")
+    e = np.loadtxt(out / "eedf.txt", skiprows=1)
+    assert per.shape == (8, 1 + len(e))
+    np.testing.assert_allclose(per[:, 0], phases, rtol=1e-14)
+    de = e[1, 0] - e[0, 0]
+    for row in per[:, 1:]:
+        assert abs(np.sum(row * np.sqrt(e[:, 0])) * de - 1.0) < 1e-6
+    rc = (out / "rateCoefficients_periodic.txt").read_text().splitlines()
+    assert rc[0] == "#" * 80
+    head = [i for i, l in enumerate(rc) if l.startswith("Phase(rad)")]
+    assert len(head) == 1 and len(rc) == head[0] + 1 + 8
+    rows = np.array([[float(x) for x in l.split()] for l in rc[head[0] + 1:]])
+    np.testing.assert_allclose(rows[:, 0], phases, rtol=1e-14)
+    np.testing.assert_allclose(rows[:, 2], np.sqrt(2.0) * 12 * np.cos(phases), rtol=1e-13, atol=1e-12)
+    tp = (out / "MCTemporalInfo_periodic.txt").read_text().splitlines()
+    assert tp[0].startswith("Phase(rad)          Phase(s)            E/N(Td)             MeanEnergy(eV)")
+    vals = np.array([[float(x) for x in l.split()] for l in tp[1:]])
+    assert vals.shape == (8, 28)
+    sw = (out / "swarmParameters.txt").read_text()
+    mean = float(re.search(r"Mean energy = (\S+) \(eV\)", sw).group(1))
+    # the per-phase mean energies average (over nearly equal counts) to the mean energy
+    assert abs(vals[:, 3].mean() - mean) < 0.02 * mean

@@ -438,3 +438,19 @@ def test_co2_with_an_elastic_cross_section_derived_from_the_effective_runs(tmp_p
     assert 0.0 < m10 < m100
     rc = (out / "reducedElecField_10" / "rateCoefficientsMC.txt").read_text()
     assert "e+CO2(X)->e+CO2(X),Elastic" in rc
+
+
+def test_rate_coefficient_ids_follow_the_reference_collision_order():
+    # Collision::ID counts collisions as they are created: the LXCat entries
+    # in file order (the Effective one included), then the Elastic derived
+    # from the Effective; the rate-coefficient files print ID + 1 and list the
+    # collisions of the flights first, then the Effective ones
+    r = dry(os.path.join(INPUT, "CO2", "CO2_swarm_setup.in"))
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    kinds = [(c["kind"], c["ref_id"], c["type"]) for c in d["collisions"]]
+    assert kinds[0] == (0, 0, "Attachment")
+    assert [k[1] for k in kinds[1:15]] == list(range(2, 16))
+    assert kinds[15] == (0, 16, "Elastic")
+    assert kinds[16] == (1, 1, "Effective")
+    assert len(kinds) == 17
