@@ -1,0 +1,104 @@
+"""The thermal-target collision branch against statistical mechanics.
+
+Without a field, electrons colliding elastically with gas atoms that move
+with the gas temperature T_g (gasTemperatureEffect: true, the branch of
+BoltzmannMC.C:916-1031 with the centre-of-mass collision of :1125-1158) relax
+to the Maxwell-Boltzmann distribution at T_g whatever their start: mean
+energy (3/2) k T_g / e and f(e) proportional to exp(-e e / (k T_g)) (the EEDF
+normalised as Int f sqrt(e) de = 1).  The ensemble starts at 10 T_g.  With cold
+targets the same gas would cool the electrons towards zero energy, so this
+equilibrium is reached only through the thermal branch.  The target mass is 20
+electron masses, so the electrons relax within tens of collisions (the
+equilibrium does not depend on the mass ratio) and the run's steady-state
+detection sees the relaxed ensemble.
+"""
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "loki-mc_amd", "bin", "lokimc_gpu")
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not os.path.exists(EXE), reason="lokimc_gpu not built")]
+
+LXCAT = """\
+PROCESS: E + A -> E + A, Elastic
+PARAM.:  m/M = 1.37e-04
+COMMENT: [e + A(gnd) -> e + A(gnd), Elastic]
+-----------------------------
+0.000000 1.00000e-19
+1000.000000 1.00000e-19
+-----------------------------
+"""
+
+SETUP = """\
+workingConditions:
+  gasPressure: 133.32
+  gasTemperature: %(T)s
+  reducedElecField: 1e-4
+  reducedMagField: 0
+  elecFieldAngle: 180
+  excitationFrequency: 0
+electronKinetics:
+  isOn: true
+  eedfType: boltzmannMC
+  ionizationOperatorType: equalSharing
+  LXCatFiles:
+    - A_LXCat.txt
+  gasProperties:
+    mass:
+      - A = 20*9.10938356e-31
+    fraction:
+      - A = 1
+  stateProperties:
+    population:
+      - A(gnd) = 1.0
+  numericsMC:
+    nElectrons: 2E4
+    gasTemperatureEffect: true
+    initialElecTempOverGasTemp: %(start)s
+    nIntegrationPoints: %(points)s
+    nIntegratedSSTimes: 2
+    nEnergyCells: 200
+output:
+  isOn: true
+  folder: thermal
+  dataFiles:
+    - eedf
+    - swarmParameters
+"""
+
+
+# (gas temperature, start temperature / T_g, integration points, bound on
+# |mean energy / (1.5 k T_g) - 1| beside 4 standard deviations)
+CASES = [(300.0, 10, "5E3", 0.005), (1000.0, 10, "5E3", 0.005), (300.0, 1, "4E4", 0.0015)]
+
+
+@pytest.mark.parametrize("T,start,points,bound", CASES)
+def test_thermal_targets_relax_electrons_to_the_gas_temperature(tmp_path, T, start, points, bound):
+    (tmp_path / "A_LXCat.txt").write_text(LXCAT)
+    (tmp_path / "t.in").write_text(SETUP % {"T": T, "start": start, "points": points})
+    r = subprocess.run([EXE, str(tmp_path / "t.in"), "--input", str(tmp_path), "--output", str(tmp_path / "Output"),
+                        "--seed", "21"], capture_output=True, text=True, timeout=600, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    out = tmp_path / "Output" / "thermal"
+    kT = 1.38064852e-23 * T / 1.6021766208e-19   # eV
+    sw = (out / "swarmParameters.txt").read_text()
+    m = re.search(r"Mean energy = (\S+) \(eV\) ; Rel. std: (\S+)%", sw)
+    mean, rel = float(m.group(1)), float(m.group(2)) / 100.0
+    print("T", T, "mean", mean, "expected", 1.5 * kT, "rel std", rel)
+    assert abs(mean - 1.5 * kT) <= 4.0 * rel * mean + bound * 1.5 * kT
+    # Maxwellian EEDF: the distribution of energies, F(e) = Int_0^e f sqrt(x) dx,
+    # against the Maxwell-Boltzmann one, a gamma(3/2, kT) distribution
+    from scipy.special import gammainc
+    e = np.loadtxt(out / "eedf.txt", skiprows=1)
+    de = e[1, 0] - e[0, 0]
+    F = np.cumsum(e[:, 1] * np.sqrt(e[:, 0]) * de)
+    edges = e[:, 0] + 0.5 * de
+    dF = np.abs(F - gammainc(1.5, edges / kT))
+    print("max |F - F_Maxwell|", dF.max())
+    assert dF.max() < 0.03
