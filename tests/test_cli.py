@@ -195,7 +195,12 @@ def test_reid_ramp_run_writes_the_reference_output_files(tmp_path):
 @pytest.mark.gpu
 def test_ac_field_run_writes_the_periodic_files(tmp_path):
     inp = _input_tree(tmp_path)
+    (inp / "extra.txt").write_text(
+        "PROCESS: E + A(gnd) -> E + A(exc2), Excitation\nPARAM.: E = 0.5 eV\nCOMMENT: [e + A(gnd) -> e + A(exc2), Excitation]\n"
+        "-----\n0.5 0\n100 1e-21\n-----\n")
     text = (REID_TEXT.replace("reducedElecField: [1,12,24]", "reducedElecField: 12")
+            .replace("  LXCatFiles:\n    - ReidRampGas/ReidRampGas_LXCat.txt",
+                     "  LXCatFiles:\n    - ReidRampGas/ReidRampGas_LXCat.txt\n  LXCatFilesExtra:\n    - extra.txt")
             .replace("excitationFrequency: 0", "excitationFrequency: 2e8")
             .replace("nElectrons: 1E5", "nElectrons: 2E4\n    nIntegrationPhases: 8")
             .replace("nIntegrationPoints: 1E4", "nIntegrationPoints: 4E3"))
@@ -218,6 +223,11 @@ def test_ac_field_run_writes_the_periodic_files(tmp_path):
     rows = np.array([[float(x) for x in l.split()] for l in rc[head[0] + 1:]])
     np.testing.assert_allclose(rows[:, 0], phases, rtol=1e-14)
     np.testing.assert_allclose(rows[:, 2], np.sqrt(2.0) * 12 * np.cos(phases), rtol=1e-13, atol=1e-12)
+    assert "# 3    e+A(gnd)->e+A(exc2),Excitation" in "\n".join(rc)
+    assert rows.shape[1] == 3 + 3     # phase, time, E/N, then R1, R2 and the extra R3
+    rates = (out / "rateCoefficients.txt").read_text()
+    assert "* Extra Rate Coefficients *" in rates and "   3 " in rates.split("* Extra Rate Coefficients *")[1]
+    assert "exc2" not in (out / "rateCoefficientsMC.txt").read_text()
     tp = (out / "MCTemporalInfo_periodic.txt").read_text().splitlines()
     assert tp[0].startswith("Phase(rad)          Phase(s)            E/N(Td)             MeanEnergy(eV)")
     vals = np.array([[float(x) for x in l.split()] for l in tp[1:]])
@@ -226,3 +236,23 @@ def test_ac_field_run_writes_the_periodic_files(tmp_path):
     mean = float(re.search(r"Mean energy = (\S+) \(eV\)", sw).group(1))
     # the per-phase mean energies average (over nearly equal counts) to the mean energy
     assert abs(vals[:, 3].mean() - mean) < 0.02 * mean
+
+
+def test_extra_lxcat_collisions_are_listed_but_do_not_fly(tmp_path):
+    # LXCatFilesExtra (Setup.h:322-347): the collisions get IDs after the
+    # regular ones and appear only in the rate-coefficient files
+    inp = _input_tree(tmp_path)
+    (inp / "extra.txt").write_text(
+        "PROCESS: E + A(gnd) -> E + A(exc2), Excitation\nPARAM.: E = 0.5 eV\nCOMMENT: [e + A(gnd) -> e + A(exc2), Excitation]\n"
+        "-----\n0.5 0\n100 1e-21\n-----\n")
+    text = REID_TEXT.replace("  LXCatFiles:\n    - ReidRampGas/ReidRampGas_LXCat.txt",
+                             "  LXCatFiles:\n    - ReidRampGas/ReidRampGas_LXCat.txt\n  LXCatFilesExtra:\n    - extra.txt")
+    assert text != REID_TEXT
+    (inp / "t.in").write_text(text)
+    r = _run([str(inp / "t.in"), "--input", str(inp), "--dry-run"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    assert len(d["processes"]) == 2
+    kinds = [(c["kind"], c["ref_id"], c["description"]) for c in d["collisions"]]
+    assert kinds == [(0, 0, "e+A(gnd)->e+A(gnd),Elastic"), (0, 1, "e+A(gnd)->e+A(exc),Excitation"),
+                     (2, 2, "e+A(gnd)->e+A(exc2),Excitation")]
