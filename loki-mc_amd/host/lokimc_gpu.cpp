@@ -220,8 +220,10 @@ int main(int argc, char** argv) {
         if (ekmc_create(&view, jobs.conds.data(), (int32_t)jobs.conds.size(), &num, &eng) != EKMC_OK) fatal(ekmc_last_error());
         if (ekmc_solve(eng) != EKMC_OK) fatal(ekmc_last_error());
         std::vector<JobData> data;
+        bool allAt180 = true;
+        for (const ekmc_condition& c : jobs.conds) allAt180 &= c.elec_field_angle == 180.0;
         for (size_t k = 0; k < jobs.conds.size(); ++k)
-            data.push_back(collectJob(eng, (int)k, jobs.conds[k], num, mix.nProcesses()));
+            data.push_back(collectJob(eng, (int)k, jobs.conds[k], num, mix.nProcesses(), allAt180));
         ekmc_destroy(eng);
         const std::string folderName = setup.value("output.folder").empty() ? "lokimc_gpu" : setup.value("output.folder");
         const std::string folder = outputDir + "/" + folderName;

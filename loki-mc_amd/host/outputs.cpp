@@ -95,7 +95,8 @@ void SwarmParams::set(const std::string& k, double v) {
     items.emplace_back(k, v);
 }
 
-JobData collectJob(const ekmc_engine* eng, int cond, const ekmc_condition& c, const ekmc_numerics& num, int nProcesses) {
+JobData collectJob(const ekmc_engine* eng, int cond, const ekmc_condition& c, const ekmc_numerics& num, int nProcesses,
+                   bool allAt180) {
     JobData j;
     j.cond = c;
     j.nElectrons = (double)c.n_electrons;
@@ -109,7 +110,7 @@ JobData collectJob(const ekmc_engine* eng, int cond, const ekmc_condition& c, co
     j.energyCells.resize(num.n_energy_cells);
     j.eedf.resize(num.n_energy_cells);
     check(ekmc_get_eedf(eng, cond, j.energyCells.data(), j.eedf.data()), "ekmc_get_eedf");
-    j.cylindrical = c.elec_field_angle == 180.0;
+    j.cylindrical = allAt180 && c.elec_field_angle == 180.0;
     if (j.cylindrical) {
         j.radialCells.resize(num.n_radial_velocity_cells);
         j.axialCells.resize(num.n_axial_velocity_cells);

@@ -31,8 +31,11 @@ struct JobData {
         fluxDiffusionPeriodic, bulkDiffusionPeriodic;
 };
 
-// Pull a condition's results out of a finished engine.
-JobData collectJob(const ekmc_engine* eng, int cond, const ekmc_condition& c, const ekmc_numerics& num, int nProcesses);
+// Pull a condition's results out of a finished engine.  `allAt180`: every
+// job of the scan has the field angle 180 degrees (the reference writes the
+// velocity and anisotropy distributions only then, WorkingConditions.h:164-172).
+JobData collectJob(const ekmc_engine* eng, int cond, const ekmc_condition& c, const ekmc_numerics& num, int nProcesses,
+                   bool allAt180);
 
 // Swarm parameters keyed as the reference's swarmParam map.
 struct SwarmParams {
