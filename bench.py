@@ -141,9 +141,10 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs one process on one MI355X; "
-              "--gpus %d is not supported (the conditions of the scan shard across GPUs "
-              "without communication, see DESIGN.md)." % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU: the engine runs one process on one MI355X; "
+              "--gpus %d is not supported (the conditions of a scan are independent and "
+              "would shard across GPUs by condition, with no communication until the "
+              "final gather of per-condition results)." % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     if args.kernel_breakdown:
