@@ -532,6 +532,10 @@ public:
         HIP_TRY(hEedf.alloc((size_t)n * nCells));
         if (anyAC) HIP_TRY(hPer.alloc((size_t)n * nPhases * nCells));
         if (anyCyl) { HIP_TRY(hEadf.alloc((size_t)n * nCells * nCos)); HIP_TRY(hEvdf.alloc((size_t)n * nRad * nAx)); }
+        // the histogram sums accumulate from zero (device memory of an earlier
+        // engine may be handed out again with its contents)
+        for (DevBuf<unsigned long long>* b : {&hEedf, &hPer, &hEadf, &hEvdf})
+            if (b->p) HIP_TRY(hipMemsetAsync(b->p, 0, b->n * sizeof(unsigned long long), stream));
         HIP_TRY(grid.alloc((size_t)n * 6));
         HIP_TRY(phaseRow.alloc(n));
         HIP_TRY(cylFlag.alloc(n));
@@ -556,6 +560,10 @@ public:
             if (flight_blocks_per_cu<V>(FIELD_DC, thermal, P, nCells) >= flight_blocks_per_cu<V>(FIELD_DC, thermal, P, 0))
                 dh.histLds = nCells;
         }
+        // otherwise the histogram kernel counts the sampled state: counting at
+        // load with HBM atomics contends on the few busy EEDF cells (measured
+        // 3.5x slower flights on the 64-condition scan)
+        if (dh.histLds < nCells) fusedHist = false;
         // first main electron of each condition, for the init kernel
         std::vector<int64_t> mb(n);
         std::vector<double> td(n);
