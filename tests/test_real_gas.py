@@ -17,6 +17,7 @@ import os
 import re
 import shutil
 import subprocess
+import tempfile
 
 import numpy as np
 import pytest
@@ -36,9 +37,10 @@ def fl16(x):
 
 
 def dry(setup, inp=INPUT, cwd=None):
-    r = subprocess.run([EXE, setup, "--input", inp, "--dry-run"], capture_output=True, text=True, timeout=300,
-                       cwd=cwd or ROOT)
-    return r
+    # a failing run writes errorLog.txt into its working directory
+    with tempfile.TemporaryDirectory() as tmp:
+        return subprocess.run([EXE, setup, "--input", inp, "--dry-run"], capture_output=True, text=True, timeout=300,
+                              cwd=cwd or tmp)
 
 
 def states_of(description):
