@@ -9,9 +9,11 @@ A step is one synchronization interval of every condition: free flights and
 collisions of all electrons to the interval end, device population control,
 ensemble moments, and the host control logic (trial frequency, tables,
 sampling).  `value` is the batched mixed-precision form (all 64 conditions in
-one set of launches); the plain FP64 one-condition-per-launch form runs the
-same conditions for comparison, and the oracle (a CPU port of the
-reference loop) is timed on a bounded sample on the host cores.
+one set of launches), with the work unit picked first by the autotuner
+(ekmc_autotune: short runs per candidate); the plain FP64
+one-condition-per-launch form runs the same conditions with its default work
+unit for comparison, and the oracle (a CPU port of the reference loop) is
+timed on a bounded sample on the host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -65,7 +67,7 @@ def run_engine(mix, conds, num, warmup, steps):
     return ev1 - ev0, real1 - real0, t1 - t0, stats
 
 
-def plain_fp64(mix, conds, base_num, warmup, steps):
+def plain_fp64(mix, conds, base_num, warmup, steps, plain_chunk=0):
     """One condition per engine (and per launch), FP64: the drop-in form."""
     events = real = 0
     seconds = 0.0
@@ -74,6 +76,7 @@ def plain_fp64(mix, conds, base_num, warmup, steps):
         num = ekmc.Numerics.from_buffer_copy(base_num)
         num.precision = ekmc.FP64
         num.seed = base_num.seed + i
+        num.work_chunk = plain_chunk  # not the batch's tuned work unit
         ev, re, sec, st = run_engine(mix, [c], num, warmup, steps)
         events += ev
         real += re
@@ -135,7 +138,11 @@ def main():
     ap.add_argument("--kernel-breakdown", action="store_true",
                     help="time every kernel (HIP events around each launch cost ~4%% per step)")
     ap.add_argument("--work-chunk", type=int, default=0,
-                    help="electrons per wavefront work unit (0: engine default)")
+                    help="electrons per wavefront work unit (0: picked by the autotuner)")
+    ap.add_argument("--plain-work-chunk", type=int, default=0,
+                    help="work unit of the plain FP64 engines (0: engine default)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="engine default work unit instead of the autotuner's pick")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
@@ -152,7 +159,17 @@ def main():
     mix, conds, num = ekmc.workload(WORKLOAD)
     num.precision = ekmc.MIXED
     num.seed = args.seed
-    num.work_chunk = args.work_chunk
+    tuning = None
+    if args.work_chunk > 0 or args.no_autotune:
+        num.work_chunk = args.work_chunk
+    else:
+        # short runs of the same batch, one per candidate work unit
+        t0 = time.perf_counter()
+        num.work_chunk, rates = ekmc.autotune(mix, conds, num)
+        tuning = {"work_chunk": num.work_chunk, "events_per_s": rates,
+                  "seconds": time.perf_counter() - t0}
+        print("bench: autotuned work unit %d (%s)" % (num.work_chunk, ", ".join(
+            "%d: %.3e" % kv for kv in rates.items())), file=sys.stderr, flush=True)
     # every condition keeps running through warmup and the timed steps, so a
     # step is always one interval of the whole 64-condition batch
     num.n_integration_points = 1e15
@@ -193,7 +210,8 @@ def main():
         "config": {"workload": WORKLOAD + ": 64 conditions, E/N 1-1000 Td x B/N {0,200,1000} Hx, "
                                "O2-like 22 channels, 1e4-1e6 electrons per condition",
                    "conditions": len(conds), "electrons": n_total, "processes": mix.n_processes,
-                   "interval": "1 / trial collision frequency"},
+                   "interval": "1 / trial collision frequency",
+                   "work_chunk": num.work_chunk or 1024},
         "real_fraction": real / events,
         "events_per_electron_interval": events / (n_total * args.steps),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -202,8 +220,10 @@ def main():
                      "bytes_per_launch": adv_bytes},
         "kernels_ms_per_step": {k: v[1] / max(args.steps, 1) for k, v in stats.items() if v[0]},
     }
+    if tuning:
+        out["autotune"] = tuning
     if not args.no_plain:
-        pe, pr, ps, padv = plain_fp64(mix, conds, num, args.plain_warmup, args.plain_steps)
+        pe, pr, ps, padv = plain_fp64(mix, conds, num, args.plain_warmup, args.plain_steps, args.plain_work_chunk)
         out["plain_fp64"] = {"value": pe / ps, "events": pe, "seconds": ps, "steps": args.plain_steps,
                              "advance_avg_ms": padv}
         out["speedup_vs_plain_fp64"] = value / (pe / ps)

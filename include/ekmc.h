@@ -231,6 +231,23 @@ ekmc_status ekmc_initialize(ekmc_engine* engine);            /* BoltzmannMC.C:42
 ekmc_status ekmc_step(ekmc_engine* engine, int32_t n_sync, int32_t* n_running);
 ekmc_status ekmc_finalize(ekmc_engine* engine);              /* BoltzmannMC.C:386-425 */
 
+/* Work-unit autotuner.  Times short runs of the given batch, one per
+ * candidate work unit of the flight kernel (main electrons per wavefront work
+ * unit: candidates / 64 electrons per lane), and returns the candidate with the
+ * most collision events per second.  Each trial creates its own engine with
+ * `numerics` except for work_chunk and the stop criteria (every condition runs
+ * through warmup + steps intervals), runs `warmup` untimed and `steps` timed
+ * intervals, and destroys it; one trial's device memory is held at a time.
+ * candidates NULL or n_candidates 0: {256, 512, 768, 1024, 1536, 2048}.
+ * rates (NULL or n_candidates doubles) receives every candidate's events/s.
+ * The work unit changes the keys of the population-control draws, so a run is
+ * reproducible for a given work_chunk, not across work units.  No counterpart
+ * in the reference: its OpenMP loop over electrons (BoltzmannMC.C:636) has no
+ * decomposition to tune. */
+ekmc_status ekmc_autotune(const ekmc_mixture* mixture, const ekmc_condition* conditions, int32_t n_cond,
+                          const ekmc_numerics* numerics, const int32_t* candidates, int32_t n_candidates,
+                          int32_t warmup, int32_t steps, int32_t* best_work_chunk, double* rates);
+
 /* ----------------------------------------------------------------------------
  * Results (the members Output<T> reads)
  * ------------------------------------------------------------------------- */

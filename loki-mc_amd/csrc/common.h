@@ -14,6 +14,7 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr int kBlock = 256;            // threads per block of every grid-wide kernel
 constexpr int kWave = 64;              // CDNA wavefront
 constexpr int kChunkDefault = 1024;    // main electrons per wave work unit (64 lanes x 16)
+constexpr int kChunkMin = 256;         // smallest default work unit (small batches)
 constexpr uint32_t kMaxLoopPerSync = 1u << 22;  // flight cap per electron per interval
 constexpr int kEnvBins = 256;          // bins of the null-test envelope of the total frequency
 

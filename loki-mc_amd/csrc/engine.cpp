@@ -257,10 +257,17 @@ public:
     DevTableSrc tsrc = {};
     DevBuf<SlotRange> ranges;
     KernelTimer timer;
+    // the cycle after the flights as a HIP graph (captured at the first
+    // interval, replayed at every later one)
+    bool useGraph = true;
+    hipGraph_t cycGraph = nullptr;
+    hipGraphExec_t cycExec = nullptr;
 
     ~Engine() override {
         if (stream) (void)hipStreamSynchronize(stream);
         set_kernel_timer(nullptr);
+        if (cycExec) (void)hipGraphExecDestroy(cycExec);
+        if (cycGraph) (void)hipGraphDestroy(cycGraph);
         for (int k = 0; k < K_COUNT; ++k) {
             if (timer.start[k]) (void)hipEventDestroy(timer.start[k]);
             if (timer.stop[k]) (void)hipEventDestroy(timer.stop[k]);
@@ -307,6 +314,13 @@ public:
         fusedHist = !(fh && fh[0] == '0');
         const char* md = std::getenv("EKMC_MERGE_DC");
         mergeDC = !(md && md[0] == '0');
+        // The cycle after the flights is replayed from a HIP graph for a batch
+        // of conditions and launched kernel by kernel for one condition (the
+        // graph measured 0.6 % faster per interval on the 64-condition scan
+        // and 2.7 % slower on the one-condition engines of the same scan).
+        // EKMC_GRAPH=1 or 0 forces either.
+        const char* gr = std::getenv("EKMC_GRAPH");
+        useGraph = gr && gr[0] ? gr[0] != '0' : n > 1;
         // HIP-event timing of the flight kernels; EKMC_KERNEL_TIMING=all times
         // every kernel, EKMC_KERNEL_TIMING=0 none
         const char* kt = std::getenv("EKMC_KERNEL_TIMING");
@@ -363,7 +377,22 @@ public:
             ldsMax)
             return fail(EKMC_ERR_INVALID_ARGUMENT, "too many processes for the per-wave collision tallies in LDS");
         if (num.work_chunk < 0 || num.reserved != 0) return fail(EKMC_ERR_INVALID_ARGUMENT, "invalid work_chunk/reserved");
-        int64_t chunkTarget = num.work_chunk > 0 ? num.work_chunk : kChunkDefault;
+        // Default work unit: 1024 main electrons per wave where that still
+        // gives every resident wave of the flight kernel (3 blocks of 4 waves
+        // per CU) a chunk, else the batch split over those waves, at least
+        // 256 (per-chunk work and newborn regions grow as chunks shrink).
+        // On one MI355X a batch of more than 3.1 M electrons keeps 1024; the
+        // one-condition engines of the 64-condition scan (1e4-1e6 electrons)
+        // ran 24 % faster with 256 than with 1024.
+        int64_t chunkTarget = num.work_chunk;
+        if (chunkTarget == 0) {
+            int64_t total = 0;
+            for (int c = 0; c < n; ++c) total += conds[c].n_electrons;
+            const int64_t waves = (int64_t)cus * 3 * (kBlock / kWave);
+            chunkTarget = total >= waves * kChunkDefault
+                              ? kChunkDefault
+                              : std::max<int64_t>(kChunkMin, (total + waves - 1) / waves);
+        }
         chunkTarget = std::max<int64_t>(kWave, (chunkTarget + kWave - 1) / kWave * kWave);
         envShift = 0;  // envelope bins of 2^envShift table rows, at most kEnvBins of them
         while (((I - 1) >> envShift) >= kEnvBins) ++envShift;
@@ -722,9 +751,48 @@ public:
         return EKMC_OK;
     }
 
-    ekmc_status run_reduce_and_fetch() {
-        HIP_TRY(launch_finish(dw, P, stream));
-        HIP_TRY(hipMemcpyAsync(hsum.p, summary.p, hsum.n * sizeof(double), hipMemcpyDeviceToHost, stream));
+    // Fixed-order reduction of the chunk records and the download of the
+    // per-condition summaries.
+    hipError_t launch_reduce() {
+        hipError_t e = launch_finish(dw, P, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(hsum.p, summary.p, hsum.n * sizeof(double), hipMemcpyDeviceToHost, stream);
+        return e;
+    }
+
+    // The cycle that follows the flights -- population control, sampling,
+    // the reduction and the summary download -- is the same launches with the
+    // same arguments every interval (the device state and work descriptors
+    // are fixed at creation; what changes lives in device memory), so it is
+    // captured once into a HIP graph and replayed.  It holds kernels and one
+    // copy only, no memset nodes.  Per-kernel timing (EKMC_KERNEL_TIMING=all)
+    // records events between the kernels and runs them one by one.
+    ekmc_status launch_cycle() {
+        if (!useGraph || timer.level >= 2) {
+            HIP_TRY(launch_popctl<V>(ds, dw, stream));
+            HIP_TRY(launch_sample<V>(ds, dw, stream));
+            HIP_TRY(launch_reduce());
+            return EKMC_OK;
+        }
+        if (!cycExec) {
+            HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = launch_popctl<V>(ds, dw, stream);
+            if (e == hipSuccess) e = launch_sample<V>(ds, dw, stream);
+            if (e == hipSuccess) e = launch_reduce();
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(stream, &g);
+            if (e == hipSuccess) e = ec;
+            if (e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                HIP_TRY(e);
+            }
+            cycGraph = g;
+            HIP_TRY(hipGraphInstantiate(&cycExec, cycGraph, nullptr, nullptr, 0));
+        }
+        HIP_TRY(hipGraphLaunch(cycExec, stream));
+        return EKMC_OK;
+    }
+
+    ekmc_status wait_summary() {
         const auto t0 = Clock::now();
         HIP_TRY(hipStreamSynchronize(stream));
         host_add(H_WAIT, ms_since(t0));
@@ -951,7 +1019,8 @@ public:
         set_kernel_timer(&timer);
         HIP_TRY(launch_init<V>(ds, dw, totalMain, mainBase.p, thermalDev.p, stream));
         HIP_TRY(launch_sample<V>(ds, dw, stream));
-        ekmc_status st = run_reduce_and_fetch();
+        HIP_TRY(launch_reduce());
+        ekmc_status st = wait_summary();
         if (st != EKMC_OK) return st;
         for (int c = 0; c < nCond; ++c) {
             Cond& C = cs[c];
@@ -1024,11 +1093,11 @@ public:
             }
             std::memcpy(pinCond.p, hcp.data(), nCond * sizeof(CondParams));
             HIP_TRY(hipMemcpyAsync(dconds.p, pinCond.p, nCond * sizeof(CondParams), hipMemcpyHostToDevice, stream));
-            HIP_TRY(launch_interval<V>(ds, dw, dpr, tables.p, dh, fl, stream));
-            HIP_TRY(launch_sample<V>(ds, dw, stream));
-            host_add(H_PREPARE, ms_since(tPrep));
-            ekmc_status st = run_reduce_and_fetch();
+            HIP_TRY(launch_flights<V>(ds, dw, dpr, tables.p, dh, fl, stream));
+            ekmc_status st = launch_cycle();
             if (st != EKMC_OK) return st;
+            host_add(H_PREPARE, ms_since(tPrep));
+            if ((st = wait_summary()) != EKMC_OK) return st;
             const auto tPost = Clock::now();
             for (int c = 0; c < nCond; ++c) {
                 if (cs[c].finished) continue;
@@ -1374,6 +1443,51 @@ ekmc_status ekmc_create(const ekmc_mixture* mixture, const ekmc_condition* condi
 }
 
 void ekmc_destroy(ekmc_engine* engine) { delete engine; }
+
+ekmc_status ekmc_autotune(const ekmc_mixture* mixture, const ekmc_condition* conditions, int32_t n_cond,
+                          const ekmc_numerics* numerics, const int32_t* candidates, int32_t n_candidates,
+                          int32_t warmup, int32_t steps, int32_t* best_work_chunk, double* rates) {
+    static const int32_t kCandidates[] = {256, 512, 768, 1024, 1536, 2048};
+    if (!mixture || !conditions || !numerics || !best_work_chunk || n_cond <= 0)
+        return fail(EKMC_ERR_INVALID_ARGUMENT, "null argument or no conditions");
+    if (warmup < 0 || steps < 1 || n_candidates < 0)
+        return fail(EKMC_ERR_INVALID_ARGUMENT, "autotune needs warmup >= 0, steps >= 1, n_candidates >= 0");
+    if (!candidates || n_candidates == 0) {
+        candidates = kCandidates;
+        n_candidates = (int32_t)(sizeof(kCandidates) / sizeof(kCandidates[0]));
+    }
+    for (int32_t i = 0; i < n_candidates; ++i)
+        if (candidates[i] <= 0) return fail(EKMC_ERR_INVALID_ARGUMENT, "work-unit candidates must be positive");
+    double best = -1;
+    int32_t bestChunk = 0;
+    for (int32_t i = 0; i < n_candidates; ++i) {
+        ekmc_numerics nm = *numerics;
+        nm.work_chunk = candidates[i];
+        // no condition may stop inside the trial
+        nm.n_integration_points = 1e300;
+        nm.max_collisions_after_ss = 1e300;
+        nm.max_sync_points = 0;
+        ekmc_engine* raw = nullptr;
+        ekmc_status st = ekmc_create(mixture, conditions, n_cond, &nm, &raw);
+        if (st != EKMC_OK) return st;
+        std::unique_ptr<ekmc_engine> e(raw);
+        int running = 0;
+        if ((st = e->impl->initialize()) != EKMC_OK) return st;
+        if (warmup > 0 && (st = e->impl->step(warmup, &running)) != EKMC_OK) return st;
+        const uint64_t ev0 = e->impl->totalEvents;
+        const auto t0 = Clock::now();
+        if ((st = e->impl->step(steps, &running)) != EKMC_OK) return st;
+        const double seconds = ms_since(t0) * 1e-3;
+        const double rate = seconds > 0 ? (double)(e->impl->totalEvents - ev0) / seconds : 0.0;
+        if (rates) rates[i] = rate;
+        if (rate > best) {
+            best = rate;
+            bestChunk = candidates[i];
+        }
+    }
+    *best_work_chunk = bestChunk;
+    return EKMC_OK;
+}
 
 ekmc_status ekmc_initialize(ekmc_engine* e) {
     if (!e) return fail(EKMC_ERR_INVALID_ARGUMENT, "null engine");

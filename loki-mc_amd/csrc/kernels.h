@@ -143,9 +143,13 @@ inline int64_t table_block_size(int I, int P) { return (int64_t)I * (2 * P + 1) 
 template <typename V>
 hipError_t launch_init(const DevState<V>& s, const DevWork& w, int64_t totalMain,
                        const int64_t* mainBase, const double* thermalDev, hipStream_t st);
+// The flights of one interval: one launch per flight class with work.
 template <typename V>
-hipError_t launch_interval(const DevState<V>& s, const DevWork& w, const DevProcesses& pr,
-                           const V* tables, const DevHist& h, const FlightLists& fl, hipStream_t st);
+hipError_t launch_flights(const DevState<V>& s, const DevWork& w, const DevProcesses& pr,
+                          const V* tables, const DevHist& h, const FlightLists& fl, hipStream_t st);
+// Population control after the flights: the same launches every interval.
+template <typename V>
+hipError_t launch_popctl(const DevState<V>& s, const DevWork& w, hipStream_t st);
 template <typename V>
 hipError_t launch_sample(const DevState<V>& s, const DevWork& w, hipStream_t st);
 template <typename V>

@@ -1418,9 +1418,8 @@ int flight_blocks_per_cu(int fc, bool thermal, int P, int histLds) {
 }
 
 template <typename V>
-hipError_t launch_interval(const DevState<V>& s, const DevWork& w, const DevProcesses& pr, const V* tables,
-                           const DevHist& h, const FlightLists& fl, hipStream_t st) {
-    const dim3 g(chunk_grid(w.nChunks)), b(kBlock);
+hipError_t launch_flights(const DevState<V>& s, const DevWork& w, const DevProcesses& pr, const V* tables,
+                          const DevHist& h, const FlightLists& fl, hipStream_t st) {
     timer_start(K_ADVANCE, st);
     launch_advance_case<V, FIELD_DC>(s, w, pr, tables, h, fl, st);
     launch_advance_case<V, FIELD_AC>(s, w, pr, tables, h, fl, st);
@@ -1429,6 +1428,12 @@ hipError_t launch_interval(const DevState<V>& s, const DevWork& w, const DevProc
     launch_advance_case<V, FIELD_AC_B>(s, w, pr, tables, h, fl, st);
     launch_advance_case<V, FLIGHT_DC_ANY>(s, w, pr, tables, h, fl, st);
     timer_stop(K_ADVANCE, st);
+    return hipGetLastError();
+}
+
+template <typename V>
+hipError_t launch_popctl(const DevState<V>& s, const DevWork& w, hipStream_t st) {
+    const dim3 g(chunk_grid(w.nChunks)), b(kBlock);
     timer_start(K_PLAN, st);
     hipLaunchKernelGGL(k_plan, dim3(w.nCond), dim3(1024), 0, st, w);
     timer_stop(K_PLAN, st);
@@ -1505,8 +1510,9 @@ hipError_t launch_op_philox(int64_t n, const uint32_t* ctr, const uint32_t* key,
 #define EKMC_INSTANTIATE(V)                                                                                   \
     template hipError_t launch_init<V>(const DevState<V>&, const DevWork&, int64_t, const int64_t*,           \
                                        const double*, hipStream_t);                                           \
-    template hipError_t launch_interval<V>(const DevState<V>&, const DevWork&, const DevProcesses&, const V*, \
-                                           const DevHist&, const FlightLists&, hipStream_t);                  \
+    template hipError_t launch_flights<V>(const DevState<V>&, const DevWork&, const DevProcesses&, const V*,  \
+                                          const DevHist&, const FlightLists&, hipStream_t);                   \
+    template hipError_t launch_popctl<V>(const DevState<V>&, const DevWork&, hipStream_t);                    \
     template hipError_t launch_sample<V>(const DevState<V>&, const DevWork&, hipStream_t);                    \
     template hipError_t launch_hist<V>(const DevState<V>&, const DevHist&, hipStream_t);                     \
     template int flight_blocks_per_cu<V>(int, bool, int, int);                                                \

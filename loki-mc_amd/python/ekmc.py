@@ -367,6 +367,7 @@ def lib():
         L.ekmc_create.argtypes = [_p(Mixture), _p(Condition), I32, _p(Numerics), _p(C.c_void_p)]
         L.ekmc_destroy.argtypes = [C.c_void_p]
         L.ekmc_destroy.restype = None
+        L.ekmc_autotune.argtypes = [_p(Mixture), _p(Condition), I32, _p(Numerics), pI32, I32, I32, I32, pI32, pD]
         for name in ("ekmc_solve", "ekmc_initialize", "ekmc_finalize", "ekmc_reset_kernel_stats"):
             getattr(L, name).argtypes = [C.c_void_p]
         L.ekmc_step.argtypes = [C.c_void_p, I32, pI32]
@@ -398,6 +399,21 @@ def check(status):
 
 def dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def autotune(mix, conditions, numerics, candidates=None, warmup=100, steps=100):
+    """ekmc_autotune: the work unit (main electrons per wavefront work unit)
+    with the most collision events/s on short runs of this batch.  Returns
+    (best work_chunk, {candidate: events/s})."""
+    conds = list(conditions)
+    arr = (Condition * len(conds))(*conds)
+    cand = np.asarray(candidates if candidates is not None else [256, 512, 768, 1024, 1536, 2048], np.int32)
+    rates = np.zeros(len(cand))
+    best = C.c_int32()
+    check(lib().ekmc_autotune(C.byref(mix.struct), arr, len(conds), C.byref(numerics),
+                              cand.ctypes.data_as(C.POINTER(C.c_int32)), len(cand), warmup, steps,
+                              C.byref(best), dptr(rates)))
+    return best.value, {int(c): float(r) for c, r in zip(cand, rates)}
 
 
 class Engine:
