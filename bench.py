@@ -13,7 +13,9 @@ one set of launches), with the work unit picked first by the autotuner
 (ekmc_autotune: short runs per candidate); the plain FP64
 one-condition-per-launch form runs the same conditions with its default work
 unit for comparison, and the oracle (a CPU port of the reference loop) is
-timed on a bounded sample on the host cores.
+timed on a bounded sample on the host cores.  `swarm_err` compares the swarm
+parameters of 8 of the scan's conditions, run to their stop criteria in the
+mixed and the FP64 form.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -103,6 +105,68 @@ def cpu_baseline(mix, conds, num, intervals, cap):
     return events / seconds, threads, seconds
 
 
+SWARM_CONDITIONS = [5, 12, 21, 30, 40, 50, 60, 63]
+
+
+def swarm_parameters(mix, eng, c):
+    """(value, standard error) of the swarm parameters of condition c in the
+    frame of E: mean energy, flux and bulk drift along E, flux and bulk
+    transverse / longitudinal diffusion (batch-means errors), total ionization
+    and attachment rate coefficients (Poisson error of their counts)."""
+    r = eng.results(c)
+    _, counts, _, _ = eng.process_data(c)
+    out = {"mean_energy": (r.mean_energy, r.mean_energy_error),
+           "flux_drift": (r.rotated_flux_drift_velocity[2], r.rotated_flux_drift_velocity_error[2]),
+           "bulk_drift": (r.rotated_bulk_drift_velocity[2], r.rotated_bulk_drift_velocity_error[2]),
+           "flux_DT": (r.rotated_flux_diffusion[0], r.rotated_flux_diffusion_error[0]),
+           "flux_DL": (r.rotated_flux_diffusion[8], r.rotated_flux_diffusion_error[8]),
+           "bulk_DT": (r.rotated_bulk_diffusion[0], r.rotated_bulk_diffusion_error[0]),
+           "bulk_DL": (r.rotated_bulk_diffusion[8], r.rotated_bulk_diffusion_error[8])}
+    types = np.asarray(mix.arrays["type"])
+    for name, t, value in (("ion_rate", 1, r.total_ion_rate_coeff), ("att_rate", 2, r.total_att_rate_coeff)):
+        n = counts[types == t].sum()
+        if n > 0:
+            out[name] = (value, value / np.sqrt(n))
+    return out
+
+
+def swarm_err(electrons=20000, points=1500):
+    """Swarm parameters of 8 conditions of the scan (E/N 1-1000 Td, all three
+    B/N), run to their stop criteria as one batch in each form, mixed
+    precision against FP64 (independent Philox keys): the deviation of each
+    parameter in combined standard errors and relative to FP64."""
+    res = []
+    for prec, seed in ((ekmc.FP64, 21), (ekmc.MIXED, 22)):
+        mix, conds, num = ekmc.workload(WORKLOAD, n_electrons=electrons, conditions=SWARM_CONDITIONS)
+        num.precision = prec
+        num.seed = seed
+        num.initial_elec_temp_over_gas_temp = 38.6817
+        num.max_collisions_before_ss = 30
+        num.n_integration_points = points
+        eng = ekmc.Engine(mix, conds, num)
+        eng.solve()
+        res.append([swarm_parameters(mix, eng, c) for c in range(len(conds))])
+        eng.close()
+    sig, rel = [], []
+    worst = None
+    for i, (a, b) in enumerate(zip(*res)):
+        for k in a:
+            if k not in b:
+                continue
+            (va, ea), (vb, eb) = a[k], b[k]
+            z = abs(vb - va) / np.hypot(ea, eb)
+            sig.append(z)
+            rel.append(abs(vb - va) / abs(va))
+            if worst is None or z > worst[0]:
+                worst = (z, SWARM_CONDITIONS[i], k)
+    return {"conditions": SWARM_CONDITIONS, "electrons": electrons, "integration_points": points,
+            "comparisons": len(sig), "max_sigma": max(sig), "median_sigma": float(np.median(sig)),
+            "max_rel": max(rel), "median_rel": float(np.median(rel)),
+            "worst": {"condition": worst[1], "parameter": worst[2]},
+            "measure": "|mixed - fp64| of each swarm parameter in combined standard errors and relative "
+                       "to fp64, independent seeds"}
+
+
 def calibrate_copy(mbytes, reps=4):
     """Device-to-device copies of a known size (the runtime's blit kernel), so
     that HBM counters of a profile can be scaled by what they read for a
@@ -131,6 +195,7 @@ def main():
     ap.add_argument("--plain-warmup", type=int, default=100)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-swarm", action="store_true", help="skip the mixed-vs-FP64 swarm comparison")
     ap.add_argument("--cpu-intervals", type=int, default=60)
     ap.add_argument("--cpu-cap", type=int, default=20000)
     ap.add_argument("--seed", type=int, default=2026)
@@ -228,6 +293,10 @@ def main():
                              "advance_avg_ms": padv}
         out["speedup_vs_plain_fp64"] = value / (pe / ps)
         print("bench: plain fp64 %.3e events/s" % (pe / ps), file=sys.stderr, flush=True)
+    if not args.no_swarm:
+        out["swarm_err"] = swarm_err()
+        print("bench: swarm parameters mixed vs fp64: max %.2f sigma, median %.2f sigma" %
+              (out["swarm_err"]["max_sigma"], out["swarm_err"]["median_sigma"]), file=sys.stderr, flush=True)
     if not args.no_cpu:
         v, threads, secs_cpu = cpu_baseline(mix, conds, num, args.cpu_intervals, args.cpu_cap)
         out["cpu_baseline"] = {"value": v, "unit": "events/s", "cores": threads, "kind": "port",
