@@ -4,12 +4,15 @@
 // GPU, and writes the output files of Output.h into Output/<folder>.
 //
 //   lokimc_gpu <setup file> [--input DIR] [--output DIR] [--precision fp64|mixed]
-//              [--seed N] [--dry-run]
+//              [--seed N] [--autotune] [--dry-run]
 //
 // --input is the folder the setup's relative paths start from (the
 // reference's Input/), --output the folder holding the output folder (the
-// reference's Output/).  --dry-run parses everything and prints the jobs and
-// the process list as JSON without touching the GPU.
+// reference's Output/).  --autotune times short runs of the scan per
+// candidate work unit (ekmc_autotune) and runs the scan with the fastest;
+// without it the engine's default work unit is used.  --dry-run parses
+// everything and prints the jobs and the process list as JSON without
+// touching the GPU.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -180,7 +183,7 @@ void dryRun(const Jobs& jobs, const ekmc_numerics& n, const GasMixture& m) {
 int main(int argc, char** argv) {
     std::string setupPath, inputDir = "Input", outputDir = "Output", precision = "fp64";
     uint64_t seed = 0;
-    bool dry = false;
+    bool dry = false, autotune = false;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&]() -> std::string {
@@ -192,11 +195,13 @@ int main(int argc, char** argv) {
         else if (a == "--precision") precision = next();
         else if (a == "--seed") seed = std::stoull(next());
         else if (a == "--dry-run") dry = true;
+        else if (a == "--autotune") autotune = true;
         else if (!a.empty() && a[0] == '-') fatal("unknown option " + a);
         else setupPath = a;
     }
     if (setupPath.empty()) {
-        std::fprintf(stderr, "usage: lokimc_gpu <setup file> [--input DIR] [--output DIR] [--precision fp64|mixed] [--seed N] [--dry-run]\n");
+        std::fprintf(stderr, "usage: lokimc_gpu <setup file> [--input DIR] [--output DIR] [--precision fp64|mixed] [--seed N] "
+                     "[--autotune] [--dry-run]\n");
         return 2;
     }
     try {
@@ -216,6 +221,14 @@ int main(int argc, char** argv) {
             return 0;
         }
         const ekmc_mixture view = mix.view();
+        if (autotune) {
+            int32_t best = 0;
+            if (ekmc_autotune(&view, jobs.conds.data(), (int32_t)jobs.conds.size(), &num, nullptr, 0, 100, 100,
+                              &best, nullptr) != EKMC_OK)
+                fatal(ekmc_last_error());
+            num.work_chunk = best;
+            std::printf("autotuned work unit: %d electrons\n", best);
+        }
         ekmc_engine* eng = nullptr;
         if (ekmc_create(&view, jobs.conds.data(), (int32_t)jobs.conds.size(), &num, &eng) != EKMC_OK) fatal(ekmc_last_error());
         if (ekmc_solve(eng) != EKMC_OK) fatal(ekmc_last_error());
