@@ -239,7 +239,8 @@ ekmc_status ekmc_finalize(ekmc_engine* engine);              /* BoltzmannMC.C:38
  * through warmup + steps intervals), runs `warmup` untimed and `steps` timed
  * intervals, and destroys it; one trial's device memory is held at a time.
  * candidates NULL or n_candidates 0: {256, 512, 768, 1024, 1536, 2048}.
- * rates (NULL or n_candidates doubles) receives every candidate's events/s.
+ * rates, NULL or one double per candidate (6 with the default candidates),
+ * receives every candidate's events/s.
  * The work unit changes the keys of the population-control draws, so a run is
  * reproducible for a given work_chunk, not across work units.  No counterpart
  * in the reference: its OpenMP loop over electrons (BoltzmannMC.C:636) has no
