@@ -4,8 +4,8 @@
  * C API of the CPU FP64 restatement of LoKI-MC's Monte Carlo loop
  * (Code/LoKI-MC/Sources/BoltzmannMC.C, MathFunctions.C,
  * Headers/AngularScatteringFunctions.h).  Only tests/, the smoke check and
- * the benchmark's CPU-baseline leg may load it; it is the checker of the GPU
- * engine, never a fallback for it.
+ * the benchmark's CPU-baseline leg may load it; the GPU engine is tested
+ * against it and never falls back to it.
  *
  * The reference draws its variates from a thread-local mt19937_64 seeded by
  * std::random_device (MathFunctions.C:31-43), so it cannot be replayed.  The

@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/lib/liboracle.so).
 
 TEST INFRASTRUCTURE: imported only by tests/, the smoke check and the
-benchmark's CPU-baseline leg, as the checker of the GPU engine.
+benchmark's CPU-baseline leg, to test the GPU engine against.
 """
 import ctypes as C
 import os
