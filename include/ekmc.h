@@ -160,7 +160,9 @@ typedef struct {
     int32_t sync_over_sampling;              /* 1 */
     int32_t max_sync_points;                 /* safety cap on sampling points, 0 = none */
     uint64_t seed;                           /* Philox key */
-    int32_t work_chunk;                      /* electrons per wavefront work unit, 0 = 1024 */
+    int32_t work_chunk;                      /* electrons per wavefront work unit; 0: 1024 for
+                                                batches above 3.1 M electrons, else the batch
+                                                over the resident waves, at least 256 */
     int32_t reserved;                        /* must be 0 */
 } ekmc_numerics;
 
