@@ -191,8 +191,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=300)
-    ap.add_argument("--plain-steps", type=int, default=100)
-    ap.add_argument("--plain-warmup", type=int, default=100)
+    # the plain engines are timed over the same intervals as the batch
+    ap.add_argument("--plain-steps", type=int, default=None, help="default: --steps")
+    ap.add_argument("--plain-warmup", type=int, default=None, help="default: --warmup")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-swarm", action="store_true", help="skip the mixed-vs-FP64 swarm comparison")
@@ -209,6 +210,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="engine default work unit instead of the autotuner's pick")
     args = ap.parse_args()
+    if args.plain_steps is None:
+        args.plain_steps = args.steps
+    if args.plain_warmup is None:
+        args.plain_warmup = args.warmup
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
