@@ -13,6 +13,14 @@
  * uniforms indexed by slot (EKMC_VARIATES_PER_EVENT per event), or the
  * Philox4x32-10 stream keyed by (seed, electron stream, event, slot) that the
  * GPU engine uses.  Everything else follows the reference's arithmetic.
+ *
+ * Parity unpinned: the serial population control inside orc_solve and
+ * orc_run_intervals (nonParallelCollisionTasks, BoltzmannMC.C:1282-1408) has
+ * no fixture or known answer (the reference cannot be built or run here); it
+ * is checked only through its invariants (N constant, the power balance with
+ * the growth term) and the whole-loop statistics.  The linear interpolation
+ * follows GSL's published gsl_interp_linear, whose installed version the
+ * reference does not pin.
  */
 #ifndef EKMC_ORACLE_H
 #define EKMC_ORACLE_H
