@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define EKMC_ABI_VERSION 1
+#define EKMC_ABI_VERSION 2
 
 /* ----------------------------------------------------------------------------
  * Status codes (replace Message::error, Message.C:6-16)
@@ -312,6 +312,9 @@ typedef struct {
     int32_t n_sampling_points, n_integration_points;
     int32_t finished;
     int32_t pad;
+    uint64_t escape_events;       /* trial events whose trial frequency was below the total
+                                     collision frequency at the electron's energy (0 when the
+                                     bound of BoltzmannMC.C:716-802 held) */
 } ekmc_cond_status;
 ekmc_status ekmc_get_status(const ekmc_engine* engine, int32_t cond, ekmc_cond_status* out);
 
@@ -382,6 +385,22 @@ ekmc_status ekmc_op_collide(ekmc_engine* engine, int32_t cond, double max_energy
                             double* vel, double* energy, const double* trial_freq,
                             const double* variates, int32_t* process, double* ejected_vel,
                             double* ejected_energy, double* energy_change);
+
+/* One synchronization interval of every running condition, as
+ * ekmc_step(engine, 1, ...) runs it, with a copy of condition `cond`'s
+ * electron pool taken after the flights and before population control: the
+ * passes of BoltzmannMC.C:630-681 without nonParallelCollisionTasks, i.e.
+ * every main electron and every electron ejected during the interval at
+ * t_sync, attached ones removed.  Every alive slot of the condition (main
+ * slots, then the newborn regions), in slot order: positions and velocities
+ * [3n], energies [n], clocks [n], Philox streams [n] and event counters [n].
+ * *n_pool: capacity in, electrons out (EKMC_ERR_CAPACITY, with the count, when
+ * the capacity is smaller).  *t_sync and *trial_freq receive the interval's
+ * end and trial frequency.  The interval then completes as in ekmc_step.
+ * Any output pointer may be NULL. */
+ekmc_status ekmc_op_interval_pool(ekmc_engine* engine, int32_t cond, int64_t* n_pool, double* pos, double* vel,
+                                  double* energy, double* time, uint64_t* stream, uint32_t* event,
+                                  double* t_sync, double* trial_freq);
 
 /* Philox4x32-10 as the engine uses it (counter [4n], key [2n] -> out [4n]). */
 ekmc_status ekmc_op_philox(int64_t n, const uint32_t* counter, const uint32_t* key,

@@ -149,6 +149,17 @@ public:
                                   double* step) = 0;
     virtual ekmc_status op_collide(int c, double maxE, int64_t n, double* vel, double* e, const double* nue,
                                    const double* var, int32_t* pid, double* ejv, double* eje, double* de) = 0;
+    // A copy of one condition's electron pool taken after the flights of the
+    // next interval and before its population control (ekmc_op_interval_pool).
+    struct PoolSnap {
+        int cond = -1;
+        bool taken = false;
+        double tsync = 0, trial = 0;
+        std::vector<double> pos, vel, energy, time;
+        std::vector<uint64_t> stream;
+        std::vector<uint32_t> event;
+    };
+    PoolSnap* snap = nullptr;
     ProcessList pl;
     ekmc_numerics num;
     std::vector<Cond> cs;
@@ -1094,6 +1105,10 @@ public:
             std::memcpy(pinCond.p, hcp.data(), nCond * sizeof(CondParams));
             HIP_TRY(hipMemcpyAsync(dconds.p, pinCond.p, nCond * sizeof(CondParams), hipMemcpyHostToDevice, stream));
             HIP_TRY(launch_flights<V>(ds, dw, dpr, tables.p, dh, fl, stream));
+            if (snap && !snap->taken && !cs[snap->cond].finished) {
+                ekmc_status st = take_snapshot(*snap);
+                if (st != EKMC_OK) return st;
+            }
             ekmc_status st = launch_cycle();
             if (st != EKMC_OK) return st;
             host_add(H_PREPARE, ms_since(tPrep));
@@ -1109,6 +1124,33 @@ public:
         int run = 0;
         for (auto& C : cs) run += C.finished ? 0 : 1;
         if (running) *running = run;
+        return EKMC_OK;
+    }
+
+    // Every alive electron of condition c's slots (main slots, then the
+    // newborn regions of its chunks), in slot order.
+    ekmc_status take_snapshot(PoolSnap& sn) {
+        const int c = sn.cond;
+        const int64_t begin = hcp[c].base, end = c + 1 < nCond ? hcp[c + 1].base : capacity;
+        const size_t n = (size_t)(end - begin);
+        std::vector<Rec<V>> h(n);
+        std::vector<uint8_t> al(n);
+        HIP_TRY(hipStreamSynchronize(stream));
+        HIP_TRY(hipMemcpy(h.data(), rec.p + begin, n * sizeof(Rec<V>), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(al.data(), alive.p + begin, n, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            if (!al[i]) continue;
+            const Rec<V>& q = h[i];
+            sn.pos.insert(sn.pos.end(), {q.x, q.y, q.z});
+            sn.vel.insert(sn.vel.end(), {(double)q.vx, (double)q.vy, (double)q.vz});
+            sn.energy.push_back(host_energy(q.vx, q.vy, q.vz));
+            sn.time.push_back(q.t);
+            sn.stream.push_back(q.stream);
+            sn.event.push_back(q.event);
+        }
+        sn.tsync = cs[c].tsync;
+        sn.trial = cs[c].trial;
+        sn.taken = true;
         return EKMC_OK;
     }
 
@@ -1681,6 +1723,7 @@ ekmc_status ekmc_get_status(const ekmc_engine* e, int32_t cond, ekmc_cond_status
     out->n_sampling_points = C.nSamp;
     out->n_integration_points = C.nIntPts;
     out->finished = C.finished ? 1 : 0;
+    out->escape_events = C.escapes;
     return EKMC_OK;
 }
 
@@ -1749,6 +1792,37 @@ ekmc_status ekmc_op_tables(ekmc_engine* e, int32_t cond, double max_energy, doub
                            double* total_freq, double* max_freq, double* energy_step) {
     if (!e || cond < 0 || cond >= (int)e->impl->cs.size()) return fail(EKMC_ERR_INVALID_ARGUMENT, "bad argument");
     return e->impl->op_tables(cond, max_energy, interp, cumul, total_freq, max_freq, energy_step);
+}
+
+ekmc_status ekmc_op_interval_pool(ekmc_engine* e, int32_t cond, int64_t* n_pool, double* pos, double* vel,
+                                  double* energy, double* time, uint64_t* stream, uint32_t* event, double* t_sync,
+                                  double* trial_freq) {
+    if (!e || !n_pool || cond < 0 || cond >= (int)e->impl->cs.size() || *n_pool < 0)
+        return fail(EKMC_ERR_INVALID_ARGUMENT, "bad argument");
+    EngineBase& E = *e->impl;
+    if (!E.initialized || E.finalized) return fail(EKMC_ERR_STATE, "the engine must be initialized and not finalized");
+    if (E.cs[cond].finished) return fail(EKMC_ERR_STATE, "the condition has finished");
+    EngineBase::PoolSnap sn;
+    sn.cond = cond;
+    E.snap = &sn;
+    int running = 0;
+    ekmc_status st = E.step(1, &running);
+    E.snap = nullptr;
+    if (st != EKMC_OK) return st;
+    if (!sn.taken) return fail(EKMC_ERR_STATE, "no flights were run for the condition");
+    const int64_t m = (int64_t)sn.energy.size();
+    const int64_t cap = *n_pool;
+    *n_pool = m;
+    if (t_sync) *t_sync = sn.tsync;
+    if (trial_freq) *trial_freq = sn.trial;
+    if (cap < m) return fail(EKMC_ERR_CAPACITY, "n_pool is smaller than the pool");
+    if (pos) std::memcpy(pos, sn.pos.data(), (size_t)m * 3 * sizeof(double));
+    if (vel) std::memcpy(vel, sn.vel.data(), (size_t)m * 3 * sizeof(double));
+    if (energy) std::memcpy(energy, sn.energy.data(), (size_t)m * sizeof(double));
+    if (time) std::memcpy(time, sn.time.data(), (size_t)m * sizeof(double));
+    if (stream) std::memcpy(stream, sn.stream.data(), (size_t)m * sizeof(uint64_t));
+    if (event) std::memcpy(event, sn.event.data(), (size_t)m * sizeof(uint32_t));
+    return EKMC_OK;
 }
 
 ekmc_status ekmc_op_philox(int64_t n, const uint32_t* counter, const uint32_t* key, uint32_t* out) {

@@ -995,10 +995,13 @@ __global__ __launch_bounds__(kBlock) void k_clone(DevState<V> s, DevWork w) {
         const int64_t th = owner_of(hoff, cp.nChunks, i);
         const Chunk& hc = w.chunks[cp.chunkBegin + th];
         const int64_t dst = w.holeList[hc.first + (i - hoff[th])];
-        // uniform alive source by rejection over the main slots (BoltzmannMC.C:1368-1371)
+        // uniform alive source by rejection over the main slots (BoltzmannMC.C:1368-1371);
+        // draw k of hole i: counter {i, k >> 16, interval, clone | (k & 0xffff)},
+        // so no two of the 2^20 draws share a counter (i < 2^32)
         int64_t src = -1;
         for (uint32_t k = 0; k < (1u << 20); ++k) {
-            double u = u53(cp.seed, (uint64_t)i, (uint32_t)cp.syncIndex, P_CLONE, k & 0xffffu);
+            double u = u53(cp.seed, (uint64_t)(uint32_t)i | ((uint64_t)(k >> 16) << 32), (uint32_t)cp.syncIndex,
+                           P_CLONE, k & 0xffffu);
             int64_t cand = (int64_t)fmin(floor(u * (double)cp.nMain), (double)(cp.nMain - 1));
             if (s.alive[cp.base + cand] == 1) { src = cp.base + cand; break; }
         }

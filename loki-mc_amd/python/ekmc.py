@@ -40,6 +40,7 @@ GASTEMP = {"false": 0, "true": 1, "smartActivation": 2}
 FP64, MIXED = 0, 1
 VARIATES_PER_EVENT = 11
 NON_DEF = -123456789
+NULL_COLLISION, PARTIAL_FLIGHT = -1, -2  # pass outcomes, GeneralDefinitions.h:38-40
 
 # ----------------------------------------------------------------- structs
 _p = C.POINTER
@@ -124,7 +125,8 @@ class CondStatus(C.Structure):
                 ("max_elec_energy", C.c_double), ("table_max_energy", C.c_double),
                 ("real_collisions", C.c_uint64),
                 ("null_collisions", C.c_uint64), ("n_sampling_points", C.c_int32),
-                ("n_integration_points", C.c_int32), ("finished", C.c_int32), ("pad", C.c_int32)]
+                ("n_integration_points", C.c_int32), ("finished", C.c_int32), ("pad", C.c_int32),
+                ("escape_events", C.c_uint64)]
 
 
 class KernelStat(C.Structure):
@@ -387,6 +389,7 @@ def lib():
         L.ekmc_op_accelerate.argtypes = [C.c_void_p, I32, I64, pD, pD, pD, pD, pD, pD]
         L.ekmc_op_collide.argtypes = [C.c_void_p, I32, D, I64, pD, pD, pD, pD, pI32, pD, pD, pD]
         L.ekmc_op_philox.argtypes = [I64, pU32, pU32, pU32]
+        L.ekmc_op_interval_pool.argtypes = [C.c_void_p, I32, _p(I64), pD, pD, pD, pD, pU64, pU32, pD, pD]
         L.ekmc_op_tables.argtypes = [C.c_void_p, I32, D, pD, pD, pD, pD, pD]
         _lib = L
     return _lib
@@ -545,6 +548,27 @@ class Engine:
                                     pid.ctypes.data_as(C.POINTER(C.c_int32)), dptr(ejv), dptr(eje),
                                     dptr(de)))
         return pid, vel, energy, ejv, eje, de
+
+
+    def op_interval_pool(self, cond, capacity=None):
+        """One interval; the pool of `cond` after its flights, before
+        population control (dict of arrays), with t_sync and the trial
+        frequency of the interval."""
+        if capacity is None:
+            capacity = 4 * self.conditions[cond].n_electrons + 4096
+        out = {"pos": np.zeros(3 * capacity), "vel": np.zeros(3 * capacity), "energy": np.zeros(capacity),
+               "time": np.zeros(capacity), "stream": np.zeros(capacity, np.uint64),
+               "event": np.zeros(capacity, np.uint32)}
+        n = C.c_int64(capacity)
+        ts, tr = C.c_double(), C.c_double()
+        check(lib().ekmc_op_interval_pool(self.h, cond, C.byref(n), dptr(out["pos"]), dptr(out["vel"]),
+                                          dptr(out["energy"]), dptr(out["time"]),
+                                          out["stream"].ctypes.data_as(C.POINTER(C.c_uint64)),
+                                          out["event"].ctypes.data_as(C.POINTER(C.c_uint32)),
+                                          C.byref(ts), C.byref(tr)))
+        m = n.value
+        res = {k: (v[:3 * m].reshape(m, 3) if k in ("pos", "vel") else v[:m]) for k, v in out.items()}
+        return res, ts.value, tr.value
 
 
 def build_tables(mix, gas_density, n_interp, max_energy):

@@ -1362,6 +1362,73 @@ int orc_series(const orc_model* m, int32_t* n_points, double* times, double* mea
     return EKMC_OK;
 }
 
+int orc_population_control(orc_model* mp, int64_t n, const int32_t* pid, const double* dE, double* pos,
+                           double* energy, double* time, const double* ej_pos, const double* ej_energy,
+                           uint64_t pop_seed, double* growth, uint64_t* counters) {
+    orc_model& m = *mp;
+    for (int64_t i = 0; i < n; ++i)
+        if (pid[i] >= m.P || (pid[i] < 0 && pid[i] != EKMC_NULL_COLLISION && pid[i] != EKMC_PARTIAL_FLIGHT &&
+                              pid[i] != EKMC_NON_DEF))
+            return EKMC_ERR_INVALID_ARGUMENT;
+    const uint64_t seed0 = m.num.seed;
+    m.num.seed = pop_seed;
+    m.popDraws = 0;
+    m.nE = n;
+    m.el.assign(n, Electron{});
+    m.ej.assign(n, Electron{});
+    m.pid.assign(pid, pid + n);
+    m.dE.assign(dE, dE + n);
+    m.gainF.assign(n, 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+        Electron& e = m.el[i];
+        Electron& j = m.ej[i];
+        for (int k = 0; k < 3; ++k) { e.r[k] = pos[3 * i + k]; j.r[k] = ej_pos[3 * i + k]; }
+        e.eps = energy[i];
+        e.t = time[i];
+        e.tfree = 1.0;
+        e.nue = 1.0;
+        e.stream = (uint64_t)i;
+        j.eps = ej_energy[i];
+        j.stream = (uint64_t)(n + i);
+    }
+    m.counters.assign(m.P, 0);
+    m.gainP.assign(m.P, 0.0);
+    m.lossP.assign(m.P, 0.0);
+    m.energyGainField = m.energyGrowth = 0;
+    m.total = m.nullc = 0;
+    non_parallel_tasks(m);
+    for (int64_t i = 0; i < n; ++i) {
+        const Electron& e = m.el[i];
+        for (int k = 0; k < 3; ++k) pos[3 * i + k] = e.r[k];
+        energy[i] = e.eps;
+        time[i] = e.t;
+    }
+    if (growth) *growth = m.energyGrowth;
+    if (counters) for (int p = 0; p < m.P; ++p) counters[p] = m.counters[p];
+    m.num.seed = seed0;
+    return EKMC_OK;
+}
+
+int orc_check_steady_state(orc_model* mp, int32_t n, const double* times, const double* mean_energy,
+                           double time, uint64_t total_collisions) {
+    orc_model& m = *mp;
+    if (n < 1) return -1;
+    if (m.f.omega != 0) return -1;  // DC models only (no phase rows set up)
+    m.nSamp = n;
+    m.idx = n - 1;
+    m.sT.assign(times, times + n);
+    m.sMeanE.assign(mean_energy, mean_energy + n);
+    m.time = time;
+    m.total = total_collisions;
+    m.ssTime = kNonDef;
+    if (m.maxElecEnergy <= 0) m.maxElecEnergy = 1.0;
+    if (m.nCells <= 0) m.nCells = m.num.n_energy_cells;
+    m.nE = m.cond.n_electrons;  // maxCollisionsBeforeSteadyState is per electron
+    m.el.assign(m.nE, Electron{});
+    check_steady_state(m);
+    return m.ssTime != kNonDef ? 1 : 0;
+}
+
 double orc_statistical_error(const double* x, int64_t n, double n_bins) {
     std::vector<double> a(x, x + n);
     return stat_error_strided(a, 1, 0, 0, (int)n, n_bins);

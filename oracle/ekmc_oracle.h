@@ -104,6 +104,24 @@ int orc_run_intervals(orc_model* m, int n_threads, int n_sync, uint64_t* events,
 int orc_series(const orc_model* m, int32_t* n_points, double* times, double* mean_energy,
                double* mean_velocity);
 
+/* nonParallelCollisionTasks (BoltzmannMC.C:1282-1408) on a caller-given
+ * ensemble of n electrons after one pass: pid[n] the pass outcome of each
+ * electron (process id, EKMC_NULL_COLLISION, EKMC_PARTIAL_FLIGHT or
+ * EKMC_NON_DEF), dE[n] its energy change, ej_pos[3n]/ej_energy[n] the electron
+ * ejected by electron i when pid[i] is an ionization.  pos[3n], energy[n] and
+ * time[n] are updated to the ensemble after attachment refills, clones and
+ * removals; the serial draws come from the sequential Philox stream of
+ * pop_seed.  *growth receives energyGrowth, counters[P] the collision counts. */
+int orc_population_control(orc_model* m, int64_t n, const int32_t* pid, const double* dE, double* pos,
+                           double* energy, double* time, const double* ej_pos, const double* ej_energy,
+                           uint64_t pop_seed, double* growth, uint64_t* counters);
+/* checkSteadyState (BoltzmannMC.C:1787-1893) on a sampled series of n points
+ * (sampling times and mean energies) at simulated time `time` with
+ * `total_collisions` real collisions so far; returns 1 when the steady state
+ * is declared, 0 otherwise, -1 for an empty series or an AC model. */
+int orc_check_steady_state(orc_model* m, int32_t n, const double* times, const double* mean_energy,
+                           double time, uint64_t total_collisions);
+
 /* Pieces of the reference's sampling/control arithmetic for unit tests. */
 double orc_statistical_error(const double* x, int64_t n, double n_bins); /* MathFunctions.C:211-231 */
 void orc_histogram(const double* x, int64_t n, const double* nodes, int32_t n_nodes,

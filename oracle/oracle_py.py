@@ -54,6 +54,9 @@ def lib():
         L.orc_solve.argtypes = [C.c_void_p, C.c_int, _p(ekmc.Results), pD, pD, pD]
         L.orc_series.argtypes = [C.c_void_p, _p(I32), pD, pD, pD]
         L.orc_run_intervals.argtypes = [C.c_void_p, C.c_int, C.c_int, _p(U64), _p(U64), pD]
+        L.orc_population_control.argtypes = [C.c_void_p, I64, _p(I32), pD, pD, pD, pD, pD, pD, U64, pD,
+                                             _p(U64)]
+        L.orc_check_steady_state.argtypes = [C.c_void_p, I32, pD, pD, D, U64]
         L.orc_statistical_error.argtypes = [pD, I64, D]
         L.orc_statistical_error.restype = D
         L.orc_histogram.argtypes = [pD, I64, pD, I32, pD]
@@ -164,6 +167,33 @@ class Model:
         if st:
             raise RuntimeError("orc_run_intervals failed: %d" % st)
         return ev.value, real.value, sec.value
+
+    def population_control(self, pid, dE, pos, energy, time, ej_pos, ej_energy, pop_seed):
+        """One serial population-control pass on a given ensemble; returns
+        (pos, energy, time, growth, counters) after it."""
+        n = len(pid)
+        pid = np.ascontiguousarray(pid, np.int32)
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1).copy()
+        energy = np.ascontiguousarray(energy, np.float64).copy()
+        time = np.ascontiguousarray(time, np.float64).copy()
+        growth = D()
+        counters = np.zeros(self.P, np.uint64)
+        st = lib().orc_population_control(
+            self.h, n, pid.ctypes.data_as(_p(I32)), dp(np.ascontiguousarray(dE, np.float64)), dp(pos),
+            dp(energy), dp(time), dp(np.ascontiguousarray(ej_pos, np.float64).reshape(-1)),
+            dp(np.ascontiguousarray(ej_energy, np.float64)), pop_seed, C.byref(growth),
+            counters.ctypes.data_as(_p(U64)))
+        if st:
+            raise RuntimeError("orc_population_control failed: %d" % st)
+        return pos.reshape(n, 3), energy, time, growth.value, counters
+
+    def check_steady_state(self, times, mean_energy, time, total_collisions):
+        t = np.ascontiguousarray(times, np.float64)
+        e = np.ascontiguousarray(mean_energy, np.float64)
+        r = lib().orc_check_steady_state(self.h, len(t), dp(t), dp(e), time, total_collisions)
+        if r < 0 or r > 1:
+            raise RuntimeError("orc_check_steady_state failed: %d" % r)
+        return bool(r)
 
     def series(self):
         n = C.c_int32(0)

@@ -40,6 +40,40 @@ def test_struct_sizes_match_header():
         assert getattr(n, name) == getattr(ref, name), name
 
 
+ABI_STRUCTS = {"ekmc_mixture": "Mixture", "ekmc_condition": "Condition", "ekmc_numerics": "Numerics",
+               "ekmc_results": "Results", "ekmc_kernel_stat": "KernelStat", "ekmc_cond_status": "CondStatus"}
+
+
+def test_struct_layouts_match_a_c_compiler(tmp_path):
+    """sizeof and offsetof of every field of every ABI struct, as gcc lays
+    out include/ekmc.h, equal the ctypes mirrors the tests and the bench use."""
+    import subprocess
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "ekmc.h"', "int main(void) {"]
+    for cname, pyname in ABI_STRUCTS.items():
+        st = getattr(ekmc, pyname)
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for field, _ in st._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, field, cname, field))
+    lines += ["return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, stdout=subprocess.PIPE, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line.strip():
+            a, b, v = line.split()
+            got[(a, b)] = int(v)
+    assert C.sizeof(ekmc.Mixture) == got[("ekmc_mixture", "size")]
+    for cname, pyname in ABI_STRUCTS.items():
+        st = getattr(ekmc, pyname)
+        assert C.sizeof(st) == got[(cname, "size")], cname
+        for field, _ in st._fields_:
+            assert getattr(st, field).offset == got[(cname, field)], (cname, field)
+
+
 @pytest.mark.parametrize("wl", ["cfg1_atomic40_dc", "cfg3_o2like_batched", "cfg4_air_ac"])
 def test_host_tables_bit_exact_with_oracle(wl):
     mix, conds, num = ekmc.workload(wl)

@@ -1,11 +1,13 @@
 """The named workloads at their full sizes (cfg5: one condition of 1e8
-electrons and 200 processes; cfg4: 24 AC conditions of 2e5; cfg2: 8 of 1e5),
+electrons and 200 processes; cfg4: 24 AC conditions of 2e5; cfg2: 8 of 1e5;
+cfg3: the 64-condition crossed-field scan of 1.56e7 electrons),
 a few synchronization intervals each, checked through properties that hold at
 any size: every condition keeps exactly its N main electrons, all synchronized
 to the interval end with finite energies; about one trial event per electron
 and interval (the interval is 1/nu_trial and trial events form a Poisson
-process of rate nu_trial); events split into real and null; and two engines
-with the same seed give the same bits."""
+process of rate nu_trial); events split into real and null; no flight of any
+condition outran the trial frequency; and two engines with the same seed give
+the same bits."""
 import ctypes as C
 
 import numpy as np
@@ -37,7 +39,9 @@ def _run(wl, prec, intervals, seed=11):
 @pytest.mark.parametrize("wl,prec,intervals", [("cfg5_mix5_large", ekmc.MIXED, 8),
                                                ("cfg4_air_ac", ekmc.MIXED, 20),
                                                ("cfg4_air_ac", ekmc.FP64, 10),
-                                               ("cfg2_n2like_scan", ekmc.FP64, 20)])
+                                               ("cfg2_n2like_scan", ekmc.FP64, 20),
+                                               ("cfg3_o2like_batched", ekmc.MIXED, 30),
+                                               ("cfg3_o2like_batched", ekmc.FP64, 10)])
 def test_full_size_intervals_keep_every_condition_whole_and_synchronized(wl, prec, intervals):
     eng, conds = _run(wl, prec, intervals)
     n_total = sum(c.n_electrons for c in conds)
@@ -46,7 +50,11 @@ def test_full_size_intervals_keep_every_condition_whole_and_synchronized(wl, pre
     print(wl, "events per electron and interval %.4f, real fraction %.4f" % (per, real / ev))
     assert 0.9 < per < 1.1  # measured 0.988-0.999
     assert 0 < real < ev
-    for c in sorted({0, len(conds) - 1}):
+    # the trial frequency bounded the total collision frequency of every
+    # flight of every condition (no escape from the null-collision method)
+    for c in range(len(conds)):
+        assert eng.status(c).escape_events == 0, c
+    for c in sorted({0, len(conds) // 2, len(conds) - 1}):
         e, t = _energies_and_times(eng, c)
         assert len(e) == conds[c].n_electrons
         assert np.all(np.isfinite(e)) and np.all(e >= 0)
