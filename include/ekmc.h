@@ -375,6 +375,9 @@ ekmc_status ekmc_op_accelerate(ekmc_engine* engine, int32_t cond, int64_t n, con
                                const double* dt, double* pos, double* vel, double* energy,
                                double* gain);
 
+/* Most events one call of ekmc_op_collide_noalloc takes. */
+#define EKMC_NOALLOC_MAX_EVENTS 32768
+
 /* Collisions (performCollision and its callees, BoltzmannMC.C:907-1280) on the
  * device with the engine's current tables of `cond`.  Inputs: vel [3n],
  * energy [n], trial frequency used for the flight [n], variates
@@ -401,6 +404,21 @@ ekmc_status ekmc_op_collide(ekmc_engine* engine, int32_t cond, double max_energy
 ekmc_status ekmc_op_interval_pool(ekmc_engine* engine, int32_t cond, int64_t* n_pool, double* pos, double* vel,
                                   double* energy, double* time, uint64_t* stream, uint32_t* event,
                                   double* t_sync, double* trial_freq);
+
+/* ekmc_op_collide without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * same arguments, kernel and results as ekmc_op_collide, for at most
+ * EKMC_NOALLOC_MAX_EVENTS events per call (more: EKMC_ERR_CAPACITY).  The
+ * device staging of the events is a static array of the library, one per
+ * GPU (5.4 MiB, held until the process ends); every call on a GPU reuses
+ * that GPU's array, so calls that overlap on one GPU (from several threads
+ * or engines) race.  Every staged value is written by the call itself, so
+ * nothing carries over from an earlier call. */
+ekmc_status ekmc_op_collide_noalloc(ekmc_engine* engine, int32_t cond, double max_energy, int64_t n,
+                                    double* vel, double* energy, const double* trial_freq,
+                                    const double* variates, int32_t* process, double* ejected_vel,
+                                    double* ejected_energy, double* energy_change);
 
 /* Philox4x32-10 as the engine uses it (counter [4n], key [2n] -> out [4n]). */
 ekmc_status ekmc_op_philox(int64_t n, const uint32_t* counter, const uint32_t* key,

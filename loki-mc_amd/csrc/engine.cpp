@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -54,6 +55,11 @@ template <typename T> struct DevBuf {
     }
     void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
 };
+
+// Device staging of one event of the collision operator: velocity [3],
+// energy, trial frequency, variates [11], ejected velocity [3], ejected
+// energy, energy change (FP64) and the process id (int32).
+constexpr size_t kCollideStageBytes = (3 + 1 + 1 + 11 + 3 + 1 + 1) * sizeof(double) + sizeof(int32_t);
 
 // Host-side phases reported after the kernels by ekmc_get_kernel_stats.
 enum HostPhase { H_PREPARE = 0, H_WAIT, H_POST, H_TABLES, H_FLUSH, H_COUNT };
@@ -149,6 +155,12 @@ public:
                                   double* step) = 0;
     virtual ekmc_status op_collide(int c, double maxE, int64_t n, double* vel, double* e, const double* nue,
                                    const double* var, int32_t* pid, double* ejv, double* eje, double* de) = 0;
+    // The same operator with its device staging at `stage` (caller-provided
+    // device memory of at least n * kCollideStageBytes bytes); allocates nothing.
+    virtual ekmc_status op_collide_staged(int c, double maxE, int64_t n, double* vel, double* e, const double* nue,
+                                          const double* var, int32_t* pid, double* ejv, double* eje, double* de,
+                                          void* stage) = 0;
+    int device = 0;  // the HIP device the engine was created on
     // A copy of one condition's electron pool taken after the flights of the
     // next interval and before its population control (ekmc_op_interval_pool).
     struct PoolSnap {
@@ -314,6 +326,7 @@ public:
         if (nCells < 1 || nCos < 1 || nRad < 1 || nAx < 1) return fail(EKMC_ERR_INVALID_ARGUMENT, "histogram sizes must be positive");
         int dev = 0;
         if (hipGetDeviceCount(&dev) != hipSuccess || dev == 0) return fail(EKMC_ERR_NO_DEVICE, "no HIP device is visible");
+        HIP_TRY(hipGetDevice(&device));
         HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         for (int k = 0; k < K_COUNT; ++k) {
             HIP_TRY(hipEventCreate(&timer.start[k]));
@@ -1326,29 +1339,48 @@ public:
         } else if (cs[c].tab.maxEnergy <= 0) {
             return fail(EKMC_ERR_STATE, "no tables built for this condition; pass max_energy > 0");
         }
+        DevBuf<unsigned char> b;
+        HIP_TRY(b.alloc((size_t)n * kCollideStageBytes));
+        ekmc_status st = collide_on(c, n, vel, e, nuev, var, pid, ejv, eje, de, b.p);
+        b.release();
+        return st;
+    }
+
+    ekmc_status op_collide_staged(int c, double maxE, int64_t n, double* vel, double* e, const double* nuev,
+                                  const double* var, int32_t* pid, double* ejv, double* eje, double* de,
+                                  void* stage) override {
+        if (maxE > 0) {
+            ekmc_status st = interpolate(c, maxE);
+            if (st != EKMC_OK) return st;
+        } else if (cs[c].tab.maxEnergy <= 0) {
+            return fail(EKMC_ERR_STATE, "no tables built for this condition; pass max_energy > 0");
+        }
+        return collide_on(c, n, vel, e, nuev, var, pid, ejv, eje, de, stage);
+    }
+
+    // Stage the inputs at `stage` (n * kCollideStageBytes bytes of device
+    // memory: 21 FP64 values and one int32 per event, every one written by
+    // the copies or the kernel), collide, copy the outputs back.
+    ekmc_status collide_on(int c, int64_t n, double* vel, double* e, const double* nuev, const double* var,
+                           int32_t* pid, double* ejv, double* eje, double* de, void* stage) {
         CondParams cp = hcp[c];
         cp.step = cs[c].tab.step;
-        DevBuf<double> b;
-        HIP_TRY(b.alloc((size_t)n * (3 + 1 + 1 + 11 + 3 + 1 + 1)));
-        double *dV = b.p, *dE = dV + 3 * n, *dN = dE + n, *dVar = dN + n, *dEjv = dVar + 11 * n, *dEje = dEjv + 3 * n,
-               *dDe = dEje + n;
-        DevBuf<int32_t> dp;
-        HIP_TRY(dp.alloc(n));
+        double *dV = (double*)stage, *dE = dV + 3 * n, *dN = dE + n, *dVar = dN + n, *dEjv = dVar + 11 * n,
+               *dEje = dEjv + 3 * n, *dDe = dEje + n;
+        int32_t* dpp = (int32_t*)(dDe + n);
         HIP_TRY(hipMemcpyAsync(dconds.p + c, &cp, sizeof(CondParams), hipMemcpyHostToDevice, stream));
         HIP_TRY(hipMemcpyAsync(dV, vel, 3 * n * 8, hipMemcpyHostToDevice, stream));
         HIP_TRY(hipMemcpyAsync(dE, e, n * 8, hipMemcpyHostToDevice, stream));
         HIP_TRY(hipMemcpyAsync(dN, nuev, n * 8, hipMemcpyHostToDevice, stream));
         HIP_TRY(hipMemcpyAsync(dVar, var, 11 * n * 8, hipMemcpyHostToDevice, stream));
-        HIP_TRY(launch_op_collide<V>(dconds.p + c, dpr, tables.p, n, dV, dE, dN, dVar, dp.p, dEjv, dEje, dDe, stream));
+        HIP_TRY(launch_op_collide<V>(dconds.p + c, dpr, tables.p, n, dV, dE, dN, dVar, dpp, dEjv, dEje, dDe, stream));
         HIP_TRY(hipMemcpyAsync(vel, dV, 3 * n * 8, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipMemcpyAsync(e, dE, n * 8, hipMemcpyDeviceToHost, stream));
-        HIP_TRY(hipMemcpyAsync(pid, dp.p, n * 4, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipMemcpyAsync(pid, dpp, n * 4, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipMemcpyAsync(ejv, dEjv, 3 * n * 8, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipMemcpyAsync(eje, dEje, n * 8, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipMemcpyAsync(de, dDe, n * 8, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
-        b.release();
-        dp.release();
         return EKMC_OK;
     }
 };
@@ -1432,6 +1464,38 @@ void fill_results(const EngineBase& E, int c, ekmc_results* r) {
     r->stopped_by_max_collisions = C.stoppedByMax;
 }
 
+}  // namespace
+}  // namespace ekmc
+
+// ============================================= allocation-free operator
+// Staging of ekmc_op_collide_noalloc: one static device array per GPU, held
+// until the process ends.  In host code `scratch` only names the symbol; its
+// device address on the current GPU comes from hipGetSymbolAddress, once per
+// device, and only that address is handed to copies and kernels.
+namespace ekmc {
+namespace {
+constexpr size_t kScratchBytes = (size_t)EKMC_NOALLOC_MAX_EVENTS * kCollideStageBytes;
+static_assert(kScratchBytes <= ((size_t)8 << 20), "the static staging stays modest");
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+void* g_scratchPtr[kMaxDevices] = {};
+
+// Device address of `scratch` on device `dev`, which must be current.
+ekmc_status scratch_address(int dev, void** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(EKMC_ERR_INVALID_ARGUMENT, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    if (!g_scratchPtr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(EKMC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratchPtr[dev] = p;
+    }
+    *out = g_scratchPtr[dev];
+    return EKMC_OK;
+}
 }  // namespace
 }  // namespace ekmc
 
@@ -1786,6 +1850,29 @@ ekmc_status ekmc_op_collide(ekmc_engine* e, int32_t cond, double max_energy, int
     if (n == 0) return EKMC_OK;
     return e->impl->op_collide(cond, max_energy, n, vel, energy, trial_freq, variates, process, ejected_vel,
                                ejected_energy, energy_change);
+}
+
+ekmc_status ekmc_op_collide_noalloc(ekmc_engine* e, int32_t cond, double max_energy, int64_t n, double* vel,
+                                    double* energy, const double* trial_freq, const double* variates,
+                                    int32_t* process, double* ejected_vel, double* ejected_energy,
+                                    double* energy_change) {
+    if (!e || cond < 0 || cond >= (int)e->impl->cs.size() || n < 0) return fail(EKMC_ERR_INVALID_ARGUMENT, "bad argument");
+    if (n > EKMC_NOALLOC_MAX_EVENTS)
+        return fail(EKMC_ERR_CAPACITY, "ekmc_op_collide_noalloc takes at most " +
+                                           std::to_string(EKMC_NOALLOC_MAX_EVENTS) + " events per call, got " +
+                                           std::to_string(n));
+    if (n == 0) return EKMC_OK;
+    int cur = 0;
+    HIP_TRY(hipGetDevice(&cur));
+    const int dev = e->impl->device;
+    if (cur != dev) HIP_TRY(hipSetDevice(dev));
+    void* stage = nullptr;
+    ekmc_status st = scratch_address(dev, &stage);
+    if (st == EKMC_OK)
+        st = e->impl->op_collide_staged(cond, max_energy, n, vel, energy, trial_freq, variates, process, ejected_vel,
+                                        ejected_energy, energy_change, stage);
+    if (cur != dev) (void)hipSetDevice(cur);
+    return st;
 }
 
 ekmc_status ekmc_op_tables(ekmc_engine* e, int32_t cond, double max_energy, double* interp, double* cumul,

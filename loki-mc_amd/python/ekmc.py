@@ -39,6 +39,7 @@ IONIZ = {"equalSharing": 0, "oneTakesAll": 1, "usingSDCS": 2, "randomUniform": 3
 GASTEMP = {"false": 0, "true": 1, "smartActivation": 2}
 FP64, MIXED = 0, 1
 VARIATES_PER_EVENT = 11
+NOALLOC_MAX_EVENTS = 32768  # EKMC_NOALLOC_MAX_EVENTS
 NON_DEF = -123456789
 NULL_COLLISION, PARTIAL_FLIGHT = -1, -2  # pass outcomes, GeneralDefinitions.h:38-40
 
@@ -388,6 +389,7 @@ def lib():
         L.ekmc_build_tables.argtypes = [_p(Mixture), D, I32, D, pD, pD, pD, pD, pD]
         L.ekmc_op_accelerate.argtypes = [C.c_void_p, I32, I64, pD, pD, pD, pD, pD, pD]
         L.ekmc_op_collide.argtypes = [C.c_void_p, I32, D, I64, pD, pD, pD, pD, pI32, pD, pD, pD]
+        L.ekmc_op_collide_noalloc.argtypes = [C.c_void_p, I32, D, I64, pD, pD, pD, pD, pI32, pD, pD, pD]
         L.ekmc_op_philox.argtypes = [I64, pU32, pU32, pU32]
         L.ekmc_op_interval_pool.argtypes = [C.c_void_p, I32, _p(I64), pD, pD, pD, pD, pU64, pU32, pD, pD]
         L.ekmc_op_tables.argtypes = [C.c_void_p, I32, D, pD, pD, pD, pD, pD]
@@ -536,17 +538,18 @@ class Engine:
                                    dptr(mx), C.byref(step)))
         return interp.reshape(I, P), cumul.reshape(I, P), tot, mx, step.value
 
-    def op_collide(self, cond, max_energy, vel, energy, trial_freq, variates):
+    def op_collide(self, cond, max_energy, vel, energy, trial_freq, variates, noalloc=False):
+        """ekmc_op_collide, or with noalloc=True ekmc_op_collide_noalloc."""
+        fn = lib().ekmc_op_collide_noalloc if noalloc else lib().ekmc_op_collide
         n = len(energy)
         vel = np.ascontiguousarray(vel, np.float64).copy()
         energy = np.ascontiguousarray(energy, np.float64).copy()
         pid = np.zeros(n, np.int32)
         ejv, eje, de = np.zeros(3 * n), np.zeros(n), np.zeros(n)
-        check(lib().ekmc_op_collide(self.h, cond, max_energy, n, dptr(vel), dptr(energy),
-                                    dptr(np.ascontiguousarray(trial_freq, np.float64)),
-                                    dptr(np.ascontiguousarray(variates, np.float64)),
-                                    pid.ctypes.data_as(C.POINTER(C.c_int32)), dptr(ejv), dptr(eje),
-                                    dptr(de)))
+        check(fn(self.h, cond, max_energy, n, dptr(vel), dptr(energy),
+                 dptr(np.ascontiguousarray(trial_freq, np.float64)),
+                 dptr(np.ascontiguousarray(variates, np.float64)),
+                 pid.ctypes.data_as(C.POINTER(C.c_int32)), dptr(ejv), dptr(eje), dptr(de)))
         return pid, vel, energy, ejv, eje, de
 
 
