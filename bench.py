@@ -16,8 +16,8 @@ one set of launches), with the work unit picked first by the autotuner
 one-condition-per-launch form runs the same conditions with its default work
 unit for comparison, and the oracle (a CPU port of the reference loop) is
 timed on a bounded sample on the host cores.  `swarm_err` compares the swarm
-parameters of 8 of the scan's conditions, run to their stop criteria in the
-mixed and the FP64 form.
+parameters of 8 of the scan's conditions, 8 independent replicas of each, run
+to their stop criteria in the mixed and the FP64 form.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -127,35 +127,28 @@ def cpu_baseline(mix, conds, num, intervals, cap):
 
 
 SWARM_CONDITIONS = [5, 12, 21, 30, 40, 50, 60, 63]
+SWARM_NAMES = ("mean_energy", "flux_drift", "bulk_drift", "flux_DT", "flux_DL", "bulk_DT", "bulk_DL",
+               "ion_rate", "att_rate")
 
 
-def swarm_parameters(mix, eng, c):
-    """(value, standard error) of the swarm parameters of condition c in the
-    frame of E: mean energy, flux and bulk drift along E, flux and bulk
-    transverse / longitudinal diffusion (batch-means errors), total ionization
-    and attachment rate coefficients (Poisson error of their counts)."""
-    r = eng.results(c)
-    _, counts, _, _ = eng.process_data(c)
-    out = {"mean_energy": (r.mean_energy, r.mean_energy_error),
-           "flux_drift": (r.rotated_flux_drift_velocity[2], r.rotated_flux_drift_velocity_error[2]),
-           "bulk_drift": (r.rotated_bulk_drift_velocity[2], r.rotated_bulk_drift_velocity_error[2]),
-           "flux_DT": (r.rotated_flux_diffusion[0], r.rotated_flux_diffusion_error[0]),
-           "flux_DL": (r.rotated_flux_diffusion[8], r.rotated_flux_diffusion_error[8]),
-           "bulk_DT": (r.rotated_bulk_diffusion[0], r.rotated_bulk_diffusion_error[0]),
-           "bulk_DL": (r.rotated_bulk_diffusion[8], r.rotated_bulk_diffusion_error[8])}
-    types = np.asarray(mix.arrays["type"])
-    for name, t, value in (("ion_rate", 1, r.total_ion_rate_coeff), ("att_rate", 2, r.total_att_rate_coeff)):
-        n = counts[types == t].sum()
-        if n > 0:
-            out[name] = (value, value / np.sqrt(n))
-    return out
+def swarm_parameters(r):
+    """Swarm parameters of a results struct in the frame of E: mean energy,
+    flux and bulk drift along E, flux and bulk transverse / longitudinal
+    diffusion, total ionization and attachment rate coefficients."""
+    return [r.mean_energy, r.rotated_flux_drift_velocity[2], r.rotated_bulk_drift_velocity[2],
+            r.rotated_flux_diffusion[0], r.rotated_flux_diffusion[8], r.rotated_bulk_diffusion[0],
+            r.rotated_bulk_diffusion[8], r.total_ion_rate_coeff, r.total_att_rate_coeff]
 
 
-def swarm_err(electrons=20000, points=1500):
-    """Swarm parameters of 8 conditions of the scan (E/N 1-1000 Td, all three
-    B/N), run to their stop criteria as one batch in each form, mixed
-    precision against FP64 (independent Philox keys): the deviation of each
-    parameter in combined standard errors and relative to FP64."""
+def swarm_err(electrons=20000, points=1500, replicas=8, bound=1e-4):
+    """Mixed precision against FP64 on 8 conditions of the scan (E/N 1-1000
+    Td, all three B/N), each run as `replicas` independent copies (own
+    Philox keys) in one batch per form, to their stop criteria.  For every
+    condition and parameter: the relative deviation of the replica means and
+    its standard error (from the replica spread), the deviation in standard
+    errors, and whether the run resolves the `bound` relative deviation
+    (|dev| + 3 se <= bound), leaves it unresolved (3 se > bound, |z| <= 3) or
+    shows a deviation (|z| > 3)."""
     res = []
     for prec, seed in ((ekmc.FP64, 21), (ekmc.MIXED, 22)):
         mix, conds, num = ekmc.workload(WORKLOAD, n_electrons=electrons, conditions=SWARM_CONDITIONS)
@@ -164,28 +157,39 @@ def swarm_err(electrons=20000, points=1500):
         num.initial_elec_temp_over_gas_temp = 38.6817
         num.max_collisions_before_ss = 30
         num.n_integration_points = points
-        eng = ekmc.Engine(mix, conds, num)
+        eng = ekmc.Engine(mix, [c for c in conds for _ in range(replicas)], num)
         eng.solve()
-        res.append([swarm_parameters(mix, eng, c) for c in range(len(conds))])
+        res.append(np.array([swarm_parameters(eng.results(c)) for c in range(len(conds) * replicas)]).reshape(
+            len(conds), replicas, -1))
         eng.close()
-    sig, rel = [], []
-    worst = None
-    for i, (a, b) in enumerate(zip(*res)):
-        for k in a:
-            if k not in b:
+    rows, zs = [], []
+    counts = {"within_bound": 0, "unresolved": 0, "deviates": 0}
+    for i, c in enumerate(SWARM_CONDITIONS):
+        for k, name in enumerate(SWARM_NAMES):
+            a, b = res[0][i, :, k], res[1][i, :, k]
+            if a.mean() == 0:
                 continue
-            (va, ea), (vb, eb) = a[k], b[k]
-            z = abs(vb - va) / np.hypot(ea, eb)
-            sig.append(z)
-            rel.append(abs(vb - va) / abs(va))
-            if worst is None or z > worst[0]:
-                worst = (z, SWARM_CONDITIONS[i], k)
-    return {"conditions": SWARM_CONDITIONS, "electrons": electrons, "integration_points": points,
-            "comparisons": len(sig), "max_sigma": max(sig), "median_sigma": float(np.median(sig)),
-            "max_rel": max(rel), "median_rel": float(np.median(rel)),
-            "worst": {"condition": worst[1], "parameter": worst[2]},
-            "measure": "|mixed - fp64| of each swarm parameter in combined standard errors and relative "
-                       "to fp64, independent seeds"}
+            se = np.hypot(a.std(ddof=1), b.std(ddof=1)) / np.sqrt(replicas)
+            dev = (b.mean() - a.mean()) / abs(a.mean())
+            se_rel = se / abs(a.mean())
+            z = (b.mean() - a.mean()) / se if se > 0 else 0.0
+            verdict = ("deviates" if abs(z) > 3 else
+                       "within_bound" if abs(dev) + 3 * se_rel <= bound else "unresolved")
+            counts[verdict] += 1
+            zs.append(abs(z))
+            rows.append({"condition": c, "parameter": name, "rel_dev": dev, "rel_se": se_rel, "z": z,
+                         "verdict": verdict})
+    worst = max(rows, key=lambda r: abs(r["z"]))
+    return {"conditions": SWARM_CONDITIONS, "replicas": replicas, "electrons": electrons,
+            "integration_points": points, "bound": bound, "comparisons": len(rows),
+            "max_sigma": max(zs), "median_sigma": float(np.median(zs)),
+            "max_rel": max(abs(r["rel_dev"]) for r in rows),
+            "median_rel": float(np.median([abs(r["rel_dev"]) for r in rows])),
+            "median_rel_se": float(np.median([r["rel_se"] for r in rows])),
+            "verdicts": counts, "worst": {k: worst[k] for k in ("condition", "parameter", "z", "rel_dev")},
+            "parameters": rows,
+            "measure": "mixed - fp64 replica means of each swarm parameter, relative to fp64 and in standard "
+                       "errors of the difference (replica spread), independent seeds"}
 
 
 def calibrate_copy(mbytes, reps=4):
@@ -230,6 +234,9 @@ def main():
                     help="work unit of the plain FP64 engines (0: engine default)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="engine default work unit instead of the autotuner's pick")
+    ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed",
+                    help="arithmetic of the batched run (fp64: the same batch in the plain form's arithmetic, "
+                         "for profiling the FP64 flight kernel)")
     args = ap.parse_args()
     if args.plain_steps is None:
         args.plain_steps = args.steps
@@ -248,7 +255,8 @@ def main():
     if args.kernel_breakdown:
         os.environ["EKMC_KERNEL_TIMING"] = "all"
     mix, conds, num = ekmc.workload(WORKLOAD)
-    num.precision = ekmc.MIXED
+    prec = ekmc.MIXED if args.precision == "mixed" else ekmc.FP64
+    num.precision = prec
     num.seed = args.seed
     # a 1 eV start and the steady state declared at the latest after 30 real
     # collisions per electron (maxCollisionsBeforeSteadyState), as in the
@@ -284,10 +292,10 @@ def main():
     adv_avg_ms = adv_ms / adv_launches
     # algorithmic bytes of one interval's flight launches: every electron's
     # record read once and written once (table reads hit L2/MALL, not counted)
-    adv_bytes = float(n_total * interval_bytes(ekmc.MIXED))
+    adv_bytes = float(n_total * interval_bytes(prec))
     achieved = adv_bytes / (adv_avg_ms * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "advance_mixed_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", "advance_%s_traffic.json" % args.precision)
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
     out = {
@@ -301,7 +309,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 flight/scatter + fp64 clock/positions/reductions",
+        "dtype": ("fp32 flight/scatter + fp64 clock/positions/reductions" if prec == ekmc.MIXED else "fp64"),
         "data": "synthetic",
         "config": {"workload": WORKLOAD + ": 64 conditions, E/N 1-1000 Td x B/N {0,200,1000} Hx, "
                                "O2-like 22 channels, 1e4-1e6 electrons per condition",
@@ -332,8 +340,9 @@ def main():
         print("bench: plain fp64 %.3e events/s" % (pe / ps), file=sys.stderr, flush=True)
     if not args.no_swarm:
         out["swarm_err"] = swarm_err()
-        print("bench: swarm parameters mixed vs fp64: max %.2f sigma, median %.2f sigma" %
-              (out["swarm_err"]["max_sigma"], out["swarm_err"]["median_sigma"]), file=sys.stderr, flush=True)
+        print("bench: swarm parameters mixed vs fp64: max %.2f sigma, median %.2f sigma, %s" %
+              (out["swarm_err"]["max_sigma"], out["swarm_err"]["median_sigma"], out["swarm_err"]["verdicts"]),
+              file=sys.stderr, flush=True)
     if not args.no_cpu:
         v, threads, secs_cpu = cpu_baseline(mix, conds, num, args.cpu_intervals, args.cpu_cap)
         out["cpu_baseline"] = {"value": v, "unit": "events/s", "cores": threads, "kind": "port",

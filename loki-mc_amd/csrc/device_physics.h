@@ -710,7 +710,10 @@ __device__ __forceinline__ int step_a(const CondParams& c, const DevProcesses& p
         return STEP_DONE;
     }
     e.t += (double)e.tfree;
-    if (THERMAL && is_thermal(c, pr, e)) return STEP_REAL;
+    if (THERMAL && is_thermal(c, pr, e)) {
+        o.escape = false;  // only the cold null test compares the trial frequency with the total
+        return STEP_REAL;
+    }
     return cold_null_test_env(c, nutot, envHi, envLo, e, rng, o) ? STEP_REAL : STEP_NULL;
 }
 

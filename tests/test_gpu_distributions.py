@@ -45,18 +45,28 @@ def _moments(eng, c, num):
     p = eedf * np.sqrt(cells) * de
     r, a, f, f1, f2 = _evdf(eng, c, num)
     dr, dz = r[1] - r[0], a[1] - a[0]
-    w = f * (2 * np.pi * r[:, None] * dr * dz)
+    # the reference's normalisation (BoltzmannMC.C:1604): counts over
+    # sum_r(row_r dz 2 pi r_r dr), one divisor for every cell, so that
+    # sum f 2 pi r dr dz = 1 while f itself stays proportional to the cell
+    # counts (the (v_r, v_z) cells already hold the 2 pi v_r Jacobian)
+    norm = (f * (2 * np.pi * r[:, None] * dr * dz)).sum()
+    w = f / f.sum()
     R, A = np.meshgrid(r, a, indexing="ij")
     speed = np.sqrt(2 * Q_M * cells)
-    return dict(p_sum=p.sum(), e_eedf=(p * cells).sum(), w_sum=w.sum(), vz_evdf=(w * A).sum(),
+    # the anisotropies are the Legendre coefficients of the energy-angle
+    # distribution (BoltzmannMC.C:1583-1600): f1 = 3 f0 <cos theta>,
+    # f2 = 5 f0 <P2(cos theta)> per energy cell, f0 the EEDF
+    jac = np.sqrt(cells) * de
+    return dict(p_sum=p.sum(), e_eedf=(p * cells).sum(), w_sum=norm, vz_evdf=(w * A).sum(),
                 e_evdf=(w * 0.5 * (R ** 2 + A ** 2)).sum() / Q_M, vz2=(w * A ** 2).sum(), vr2=(w * R ** 2).sum(),
-                vz_eadf=(p * speed * f1 / 3.0).sum(), cos_mean=(p * f1 / 3.0).sum(), p2_mean=(p * f2 / 5.0).sum())
+                vz_eadf=(jac * speed * f1 / 3.0).sum(), cos_mean=(jac * f1 / 3.0).sum(),
+                p2_mean=(jac * f2 / 5.0).sum())
 
 
 @pytest.mark.parametrize("prec", [ekmc.FP64, ekmc.MIXED])
 def test_distribution_moments_reproduce_the_sampled_averages(prec):
     mix, _, _ = ekmc.workload("cfg1_atomic40_dc")
-    conds = [condition(en=100.0, n_e=20000), condition(en=300.0, n_e=20000)]
+    conds = [condition(en=100.0, n_e=20000), condition(en=150.0, n_e=20000)]
     num = ekmc.numerics_default()
     num.precision = prec
     num.seed = 77

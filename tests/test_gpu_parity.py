@@ -495,38 +495,49 @@ def test_cfg1_swarm_statistics_match_oracle():
     eng.close()
 
 
+def _replica_swarm(r):
+    """Swarm parameters along E (rotated frame) and the total ionization and
+    attachment rate coefficients of one results struct."""
+    return [r.mean_energy, r.rotated_flux_drift_velocity[2], r.rotated_bulk_drift_velocity[2],
+            0.5 * (r.rotated_flux_diffusion[0] + r.rotated_flux_diffusion[4]), r.rotated_flux_diffusion[8],
+            0.5 * (r.rotated_bulk_diffusion[0] + r.rotated_bulk_diffusion[4]), r.rotated_bulk_diffusion[8],
+            r.total_ion_rate_coeff, r.total_att_rate_coeff]
+
+
+REPLICA_NAMES = ("mean energy", "flux drift", "bulk drift", "flux D_T", "flux D_L", "bulk D_T", "bulk D_L",
+                 "ionization rate", "attachment rate")
+
+
 def test_mixed_precision_matches_fp64_statistically():
     """Every swarm parameter of the mixed form against the FP64 form on an
-    attachment-dominated (52 Td) and an ionization-dominated (1000 Td,
-    crossed B) condition of the O2-like scan, independent seeds: mean energy,
-    flux and bulk drift, flux and bulk transverse and longitudinal diffusion
-    within three combined batch-means standard errors; total ionization and
-    attachment rate coefficients within three combined standard errors from
-    8 replicas of each form."""
-    mix, conds, _ = ekmc.workload("cfg3_o2like_batched", n_electrons=20000, conditions=[12, 21])
-    R = 8
+    attachment-dominated (52 Td), an ionization-dominated (1000 Td) and a
+    crossed-field (1000 Td, 1000 Hx) condition of the O2-like scan: 16
+    replicas of each condition per form in one batch (independent Philox
+    keys); the replica means differ by less than three standard errors of
+    their difference, for mean energy, flux and bulk drift, flux and bulk
+    transverse and longitudinal diffusion and the total ionization and
+    attachment rate coefficients."""
+    idx = [12, 21, 63]
+    mix, conds, _ = ekmc.workload("cfg3_o2like_batched", n_electrons=20000, conditions=idx)
+    R = 16
     res = {}
     for prec, seed in ((ekmc.FP64, 21), (ekmc.MIXED, 22)):
         num = _short_numerics(seed, prec)
         num.n_integration_points = 1500
         eng = _engine(mix, [c for c in conds for _ in range(R)], num)
         eng.solve()
-        res[prec] = ([eng.results(c) for c in range(len(conds) * R)],
-                     _replica_rates(eng, range(len(conds) * R), mix.arrays["type"]))
+        res[prec] = np.array([_replica_swarm(eng.results(c)) for c in range(len(conds) * R)]).reshape(
+            len(conds), R, -1)
         eng.close()
-    for ci in range(len(conds)):
-        a, b = res[ekmc.FP64][0][ci * R], res[ekmc.MIXED][0][ci * R]
-        for (name, x, ex), (_, y, ey) in zip(_swarm(a, True), _swarm(b, True)):
-            print(ci, name, x, ex, y, ey)
-            assert _combined_sigma(x, ex, y, ey), (ci, name)
-        for k, what in ((0, "ionization"), (1, "attachment")):
-            ra = res[ekmc.FP64][1][ci * R:(ci + 1) * R, k]
-            rb = res[ekmc.MIXED][1][ci * R:(ci + 1) * R, k]
-            if ra.mean() == 0 and rb.mean() == 0:
+    for ci, c in enumerate(idx):
+        a, b = res[ekmc.FP64][ci], res[ekmc.MIXED][ci]
+        for k, name in enumerate(REPLICA_NAMES):
+            if a[:, k].mean() == 0 and b[:, k].mean() == 0:
                 continue
-            se = np.hypot(ra.std(ddof=1), rb.std(ddof=1)) / np.sqrt(R)
-            print(ci, what, ra.mean(), rb.mean(), se)
-            assert abs(ra.mean() - rb.mean()) <= 3.0 * se, (ci, what)
+            se = np.hypot(a[:, k].std(ddof=1), b[:, k].std(ddof=1)) / np.sqrt(R)
+            z = (b[:, k].mean() - a[:, k].mean()) / se
+            print(c, name, a[:, k].mean(), b[:, k].mean(), "z %.2f" % z)
+            assert abs(z) <= 3.0, (c, name, z)
 
 
 def test_reid_ramp_matches_published_loki_mc_values():

@@ -153,7 +153,9 @@ def test_mixed_samplers_follow_oracle_to_fp32_accuracy():
     vg3, vo3 = vg.reshape(n, 3)[same], vo.reshape(n, 3)[same]
     cg, co = _cos_between(v[same], vg3), _cos_between(v[same], vo3)
     assert np.all(np.abs(cg - co) <= 2e-4), np.abs(cg - co).max()
-    np.testing.assert_allclose(eg[same], eo[same], rtol=2e-5, atol=1e-6)
+    # the mixed form's incident energy comes from FP32 velocities (a few
+    # 1e-8 relative); the energy left after a threshold is kept in FP64
+    assert np.all(np.abs(eg[same] - eo[same]) <= 2e-5 * eo[same] + 3e-7 * e[same])
     ion = same & (ejeo > 1e-3)
     ce_g = _cos_between(v[ion], ejvg.reshape(n, 3)[ion])
     ce_o = _cos_between(v[ion], ejvo.reshape(n, 3)[ion])
