@@ -610,7 +610,10 @@ public:
         dh.histLds = 0;
         if (nCells <= 2048) {
             const bool thermal = num.gas_temperature_effect != EKMC_GASTEMP_FALSE;
-            if (flight_blocks_per_cu<V>(FIELD_DC, thermal, P, nCells) >= flight_blocks_per_cu<V>(FIELD_DC, thermal, P, 0))
+            DevHist withLds = dh, noLds = dh;
+            withLds.histLds = nCells;
+            noLds.histLds = 0;
+            if (flight_blocks_per_cu<V>(FIELD_DC, thermal, P, withLds) >= flight_blocks_per_cu<V>(FIELD_DC, thermal, P, noLds))
                 dh.histLds = nCells;
         }
         // otherwise the histogram kernel counts the sampled state: counting at
@@ -764,7 +767,7 @@ public:
             fl.list[fc] = flist.p + begin;
             fl.count[fc] = count;
             if (count) {
-                const int64_t waves = (int64_t)flight_blocks_per_cu<V>(fc, thermal, P, dh.histLds) * cus;
+                const int64_t waves = (int64_t)flight_blocks_per_cu<V>(fc, thermal, P, dh) * cus;
                 const int64_t need = (count + kBlock / kWave - 1) / (kBlock / kWave);
                 fl.grid[fc] = (int32_t)std::max<int64_t>(1, std::min<int64_t>(need, waves));
             }

@@ -156,8 +156,9 @@ template <typename V>
 hipError_t launch_hist(const DevState<V>& s, const DevHist& h, hipStream_t st);
 hipError_t launch_finish(const DevWork& w, int P, hipStream_t st);
 // Resident blocks per CU of the flight kernel of field case (or class) fc.
+// (histogram configuration h: the EEDF in LDS when h.histLds >= h.nCells)
 template <typename V>
-int flight_blocks_per_cu(int fc, bool thermal, int P, int histLds);
+int flight_blocks_per_cu(int fc, bool thermal, int P, const DevHist& h);
 // Record ring of the flight kernel: segments of kRingSeg main records per
 // wave, ring_segs<V>() of them (the mixed record is smaller, so its ring is
 // deeper in the same LDS).

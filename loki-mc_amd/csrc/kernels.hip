@@ -384,19 +384,25 @@ __device__ void wave_refill(const DevState<V>& s, uint32_t* holeList, const Chun
 // MathFunctions.C:61-81,106-127): the EEDF into `lhist` (LDS, flushed later)
 // or directly into HBM (phase row `row`, or -1), the EADF and EVDF into HBM
 // when `cyl`.  The per-condition values are read once per chunk by the caller.
-template <typename V>
+// Histogram modes of a flight launch, fixed per launch so that the paths a
+// launch cannot take are not compiled into it: the EEDF in LDS and no
+// cylindrically symmetric condition, the EEDF in LDS and EADF/EVDF in HBM,
+// or everything in HBM.
+enum HistMode { HIST_LDS = 0, HIST_LDS_CYL = 1, HIST_HBM = 2 };
+
+template <typename V, int HIST>
 __device__ __forceinline__ void hist_count(const DevHist& h, int c, const double* g, const Electron<V>& e,
                                            uint32_t* lhist, int row, bool cyl) {
     const double en = (double)e.eps;
     const int cell = (int)(en / g[0]);
     if (cell >= 0 && cell < h.nCells) {
-        if (lhist) lds_add_u32(lhist + cell, 1u);
+        if (HIST != HIST_HBM) lds_add_u32(lhist + cell, 1u);
         else {
             atomicAdd(&h.eedf[(size_t)c * h.nCells + cell], 1ull);
             if (row >= 0 && h.eedfPer) atomicAdd(&h.eedfPer[((size_t)c * h.nPhases + row) * h.nCells + cell], 1ull);
         }
     }
-    if (cyl) {
+    if (HIST != HIST_LDS && cyl) {
         const double vx = e.vx, vy = e.vy, vz = e.vz;
         const double cosA = vz / sqrt(vx * vx + vy * vy + vz * vz);
         const int cy = (int)((cosA + 1.0) / g[1]);
@@ -412,7 +418,7 @@ __device__ __forceinline__ void hist_count(const DevHist& h, int c, const double
 // --------------------------------------------------------------- advance
 // One chunk: every main electron and every newborn flies to t_sync (the
 // passes of BoltzmannMC.C:630-681 for this set of electrons).
-template <typename V, int FC, bool THERMAL>
+template <typename V, int FC, bool THERMAL, int HIST>
 __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWork& w, const DevProcesses& pr,
                                               const V* tables, const DevHist& h, int cid, unsigned long long* cnt,
                                               lds_uint4* ring) {
@@ -438,9 +444,9 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
     // the condition's grid is copied to LDS
     if (lane < 6) hg[lane] = h.grid[6 * (size_t)c + lane];
     const bool histOn = h.grid[6 * (size_t)c + 5] == 2.0;
-    const bool histLds = histOn && h.histLds >= h.nCells;
+    const bool histLds = HIST != HIST_HBM && histOn;
     const int histRow = histOn ? h.phaseRow[c] : -1;
-    const bool histCyl = histOn && h.eadf && h.cyl[c] != 0;
+    const bool histCyl = HIST != HIST_LDS && histOn && h.eadf && h.cyl[c] != 0;
     if (histLds)
         for (int i = lane; i < h.nCells; i += kWave) lhist[i] = 0;
     wave_lds_sync();
@@ -496,7 +502,7 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
                         ring_read<V>(ring_segment<V>(ring, idx), idx % kRingSeg, e);
                     }
                     iter = 0;
-                    if (histOn && !isNew) hist_count<V>(h, c, hg, e, histLds ? lhist : nullptr, histRow, histCyl);
+                    if (histOn && !isNew) hist_count<V, HIST>(h, c, hg, e, lhist, histRow, histCyl);
                 }
             }
             nextMain += takeMain;
@@ -644,7 +650,7 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
 
 // Persistent flight kernel of one field case: every wave takes chunks from
 // the list through an atomic counter until the list is exhausted.
-template <typename V, int FC, bool THERMAL>
+template <typename V, int FC, bool THERMAL, int HIST>
 __global__ __launch_bounds__(kBlock) void k_advance(DevState<V> s, DevWork w, DevProcesses pr, const V* tables,
                                                     DevHist h, const int32_t* list, int32_t nList,
                                                     uint32_t* queue) {
@@ -668,7 +674,7 @@ __global__ __launch_bounds__(kBlock) void k_advance(DevState<V> s, DevWork w, De
         if (lane_id() == 0) item = atomicAdd(queue, 1u);
         item = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl((int)item, 0, kWave));
         if (item >= (uint32_t)nList) break;
-        advance_chunk<V, FC, THERMAL>(s, w, prl, tables, h, list[item], cnt, ring);
+        advance_chunk<V, FC, THERMAL, HIST>(s, w, prl, tables, h, list[item], cnt, ring);
     }
 }
 
@@ -1386,37 +1392,69 @@ hipError_t launch_init(const DevState<V>& s, const DevWork& w, int64_t totalMain
     return hipGetLastError();
 }
 
+// Histogram mode of the flight launches of an engine: the EEDF in LDS when
+// the engine counts it there, EADF/EVDF code only when a condition is
+// cylindrically symmetric.
+static inline int hist_mode(const DevHist& h) {
+    if (h.histLds < h.nCells) return HIST_HBM;
+    return h.eadf ? HIST_LDS_CYL : HIST_LDS;
+}
+
 template <typename V, int FC, bool TH>
-static const void* flight_kernel() { return (const void*)k_advance<V, FC, TH>; }
+static const void* flight_kernel(int hist) {
+    if (hist == HIST_LDS) return (const void*)k_advance<V, FC, TH, HIST_LDS>;
+    if (hist == HIST_LDS_CYL) return (const void*)k_advance<V, FC, TH, HIST_LDS_CYL>;
+    return (const void*)k_advance<V, FC, TH, HIST_HBM>;
+}
+
+template <typename V, int FC, bool TH, int HIST>
+static void launch_advance_hist(const DevState<V>& s, const DevWork& w, const DevProcesses& pr, const V* tables,
+                                const DevHist& h, const FlightLists& fl, hipStream_t st) {
+    dim3 g(fl.grid[FC]), b(kBlock);
+    const size_t lds = flight_lds<V>(pr.P, h.histLds);
+    hipLaunchKernelGGL((k_advance<V, FC, TH, HIST>), g, b, lds, st, s, w, pr, tables, h, fl.list[FC], fl.count[FC],
+                       w.queue + FC);
+}
+
+template <typename V, int FC, bool TH>
+static void launch_advance_th(const DevState<V>& s, const DevWork& w, const DevProcesses& pr, const V* tables,
+                              const DevHist& h, const FlightLists& fl, hipStream_t st) {
+    switch (hist_mode(h)) {
+        case HIST_LDS: launch_advance_hist<V, FC, TH, HIST_LDS>(s, w, pr, tables, h, fl, st); break;
+        case HIST_LDS_CYL: launch_advance_hist<V, FC, TH, HIST_LDS_CYL>(s, w, pr, tables, h, fl, st); break;
+        default: launch_advance_hist<V, FC, TH, HIST_HBM>(s, w, pr, tables, h, fl, st); break;
+    }
+}
 
 template <typename V, int FC>
 static void launch_advance_case(const DevState<V>& s, const DevWork& w, const DevProcesses& pr, const V* tables,
                                 const DevHist& h, const FlightLists& fl, hipStream_t st) {
     if (fl.count[FC] == 0) return;
-    dim3 g(fl.grid[FC]), b(kBlock);
-    const size_t lds = flight_lds<V>(pr.P, h.histLds);
-    uint32_t* q = w.queue + FC;
-    if (pr.gasTempEffect == 0)
-        hipLaunchKernelGGL((k_advance<V, FC, false>), g, b, lds, st, s, w, pr, tables, h, fl.list[FC], fl.count[FC], q);
-    else
-        hipLaunchKernelGGL((k_advance<V, FC, true>), g, b, lds, st, s, w, pr, tables, h, fl.list[FC], fl.count[FC], q);
+    if (pr.gasTempEffect == 0) launch_advance_th<V, FC, false>(s, w, pr, tables, h, fl, st);
+    else launch_advance_th<V, FC, true>(s, w, pr, tables, h, fl, st);
 }
 
 template <typename V>
-int flight_blocks_per_cu(int fc, bool thermal, int P, int histLds) {
+int flight_blocks_per_cu(int fc, bool thermal, int P, const DevHist& h) {
     const void* k = nullptr;
+    const int hm = hist_mode(h);
     switch (fc) {
-        case FIELD_DC: k = thermal ? flight_kernel<V, FIELD_DC, true>() : flight_kernel<V, FIELD_DC, false>(); break;
-        case FIELD_AC: k = thermal ? flight_kernel<V, FIELD_AC, true>() : flight_kernel<V, FIELD_AC, false>(); break;
-        case FIELD_DC_B: k = thermal ? flight_kernel<V, FIELD_DC_B, true>() : flight_kernel<V, FIELD_DC_B, false>(); break;
-        case FIELD_ECR: k = thermal ? flight_kernel<V, FIELD_ECR, true>() : flight_kernel<V, FIELD_ECR, false>(); break;
-        case FLIGHT_DC_ANY:
-            k = thermal ? flight_kernel<V, FLIGHT_DC_ANY, true>() : flight_kernel<V, FLIGHT_DC_ANY, false>();
+        case FIELD_DC: k = thermal ? flight_kernel<V, FIELD_DC, true>(hm) : flight_kernel<V, FIELD_DC, false>(hm); break;
+        case FIELD_AC: k = thermal ? flight_kernel<V, FIELD_AC, true>(hm) : flight_kernel<V, FIELD_AC, false>(hm); break;
+        case FIELD_DC_B:
+            k = thermal ? flight_kernel<V, FIELD_DC_B, true>(hm) : flight_kernel<V, FIELD_DC_B, false>(hm);
             break;
-        default: k = thermal ? flight_kernel<V, FIELD_AC_B, true>() : flight_kernel<V, FIELD_AC_B, false>(); break;
+        case FIELD_ECR:
+            k = thermal ? flight_kernel<V, FIELD_ECR, true>(hm) : flight_kernel<V, FIELD_ECR, false>(hm);
+            break;
+        case FLIGHT_DC_ANY:
+            k = thermal ? flight_kernel<V, FLIGHT_DC_ANY, true>(hm) : flight_kernel<V, FLIGHT_DC_ANY, false>(hm);
+            break;
+        default: k = thermal ? flight_kernel<V, FIELD_AC_B, true>(hm) : flight_kernel<V, FIELD_AC_B, false>(hm); break;
     }
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kBlock, flight_lds<V>(P, histLds)) != hipSuccess || n < 1) n = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kBlock, flight_lds<V>(P, h.histLds)) != hipSuccess || n < 1)
+        n = 1;
     return n;
 }
 
@@ -1518,7 +1556,7 @@ hipError_t launch_op_philox(int64_t n, const uint32_t* ctr, const uint32_t* key,
     template hipError_t launch_popctl<V>(const DevState<V>&, const DevWork&, hipStream_t);                    \
     template hipError_t launch_sample<V>(const DevState<V>&, const DevWork&, hipStream_t);                    \
     template hipError_t launch_hist<V>(const DevState<V>&, const DevHist&, hipStream_t);                     \
-    template int flight_blocks_per_cu<V>(int, bool, int, int);                                                \
+    template int flight_blocks_per_cu<V>(int, bool, int, const DevHist&);                                                \
     template hipError_t launch_tables<V>(const DevTableSrc&, double, double, int, V*, double*, double*,        \
                                          hipStream_t);                                                       \
     template hipError_t launch_op_accelerate<V>(const CondParams*, int64_t, const double*, const double*,     \

@@ -14,13 +14,19 @@
  * Philox4x32-10 stream keyed by (seed, electron stream, event, slot) that the
  * GPU engine uses.  Everything else follows the reference's arithmetic.
  *
- * Parity unpinned: the serial population control inside orc_solve and
- * orc_run_intervals (nonParallelCollisionTasks, BoltzmannMC.C:1282-1408) has
- * no fixture or known answer (the reference cannot be built or run here); it
- * is checked only through its invariants (N constant, the power balance with
- * the growth term) and the whole-loop statistics.  The linear interpolation
+ * How each function is pinned (tests/): Philox by the Random123 known-answer
+ * vectors; tables by numpy interpolation, prefix sums and the Klein-Rosseland
+ * closed form; flights by ODE integration and field-work identities; channel
+ * choice by a brute-force linear scan; scattering by energy/momentum
+ * invariants and KS tests against the reference's angular distributions;
+ * population control (orc_population_control) by the exact probabilities its
+ * rules fix (last-in first-out refills, uniform clones, uniform removal
+ * subsets, the growth identity); the trial-frequency energy bound by flights
+ * that never exceed it; checkSteadyState by constructed series; the whole
+ * loop (orc_solve) by the published Reid-ramp and Lucas-Saelee results and by
+ * the sampling counts its control cadence fixes.  The linear interpolation
  * follows GSL's published gsl_interp_linear, whose installed version the
- * reference does not pin.
+ * reference does not pin (parity unpinned at that version only).
  */
 #ifndef EKMC_ORACLE_H
 #define EKMC_ORACLE_H
