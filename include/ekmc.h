@@ -160,9 +160,17 @@ typedef struct {
     int32_t sync_over_sampling;              /* 1 */
     int32_t max_sync_points;                 /* safety cap on sampling points, 0 = none */
     uint64_t seed;                           /* Philox key */
-    int32_t work_chunk;                      /* electrons per wavefront work unit; 0: 1024 for
-                                                batches above 3.1 M electrons, else the batch
-                                                over the resident waves, at least 256 */
+    int32_t work_chunk;                      /* electrons per wavefront work unit (at most
+                                                65535); 0: 1024 for batches above 3.1 M
+                                                electrons, else the batch over the resident
+                                                waves, at least 256 */
+    int32_t park_lanes;                      /* lanes of a wavefront that wait for a real
+                                                collision before it runs the collision step
+                                                for all of them (1-64; 0: 32) */
+    int32_t env_coarsen;                     /* the null-test envelope of the total collision
+                                                frequency, staged in LDS, uses bins of
+                                                2^env_coarsen times the fewest rows that fit
+                                                256 bins (0-3; 0: finest) */
     int32_t reserved;                        /* must be 0 */
 } ekmc_numerics;
 
@@ -250,6 +258,16 @@ ekmc_status ekmc_finalize(ekmc_engine* engine);              /* BoltzmannMC.C:38
 ekmc_status ekmc_autotune(const ekmc_mixture* mixture, const ekmc_condition* conditions, int32_t n_cond,
                           const ekmc_numerics* numerics, const int32_t* candidates, int32_t n_candidates,
                           int32_t warmup, int32_t steps, int32_t* best_work_chunk, double* rates);
+
+/* Launch autotuner: the work unit as ekmc_autotune does, then, at that work
+ * unit, the collision-step lane threshold (park_lanes: 16, 24, 32, 40, 48),
+ * then the envelope coarseness (env_coarsen: 0, 1, 2), each trial a short run
+ * of the batch as in ekmc_autotune.  *best receives `numerics` with the
+ * fastest work_chunk, park_lanes and env_coarsen; *best_rate (may be NULL)
+ * its collision events per second.  No counterpart in the reference. */
+ekmc_status ekmc_autotune_launch(const ekmc_mixture* mixture, const ekmc_condition* conditions, int32_t n_cond,
+                                 const ekmc_numerics* numerics, int32_t warmup, int32_t steps,
+                                 ekmc_numerics* best, double* best_rate);
 
 /* ----------------------------------------------------------------------------
  * Results (the members Output<T> reads)

@@ -59,6 +59,7 @@ struct DevProcesses {
     int gasTempEffect;       // EKMC_GASTEMP_*
     int ionizOp;             // EKMC_IONIZ_*
     double sharingFactor;    // 0.5 equalSharing, 0 oneTakesAll (BoltzmannMC.h:262-270)
+    int parkLanes;           // lanes waiting for a real collision that start the collision step
     const ProcRec* proc;     // [P]
     const int* gasFirst;     // [G]
     const int* gasLast;      // [G]

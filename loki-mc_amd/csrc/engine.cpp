@@ -324,6 +324,12 @@ public:
         nCells = num.n_energy_cells; nCos = num.n_cos_angle_cells;
         nRad = num.n_radial_velocity_cells; nAx = num.n_axial_velocity_cells;
         if (nCells < 1 || nCos < 1 || nRad < 1 || nAx < 1) return fail(EKMC_ERR_INVALID_ARGUMENT, "histogram sizes must be positive");
+        if (num.park_lanes < 0 || num.park_lanes > kWave)
+            return fail(EKMC_ERR_INVALID_ARGUMENT, "park_lanes must be 0 (default) or 1-64");
+        if (num.env_coarsen < 0 || num.env_coarsen > 3) return fail(EKMC_ERR_INVALID_ARGUMENT, "env_coarsen must be 0-3");
+        if (num.work_chunk < 0 || num.reserved != 0) return fail(EKMC_ERR_INVALID_ARGUMENT, "invalid work_chunk/reserved");
+        // the flight kernel's LDS EEDF counts a chunk's main electrons in 16 bits
+        if (num.work_chunk > 65535) return fail(EKMC_ERR_INVALID_ARGUMENT, "work_chunk must be at most 65535");
         int dev = 0;
         if (hipGetDeviceCount(&dev) != hipSuccess || dev == 0) return fail(EKMC_ERR_NO_DEVICE, "no HIP device is visible");
         HIP_TRY(hipGetDevice(&device));
@@ -400,7 +406,6 @@ public:
                 flight_ring_lds<V>() >
             ldsMax)
             return fail(EKMC_ERR_INVALID_ARGUMENT, "too many processes for the per-wave collision tallies in LDS");
-        if (num.work_chunk < 0 || num.reserved != 0) return fail(EKMC_ERR_INVALID_ARGUMENT, "invalid work_chunk/reserved");
         // Default work unit: 1024 main electrons per wave where that still
         // gives every resident wave of the flight kernel (3 blocks of 4 waves
         // per CU) a chunk, else the batch split over those waves, at least
@@ -420,6 +425,7 @@ public:
         chunkTarget = std::max<int64_t>(kWave, (chunkTarget + kWave - 1) / kWave * kWave);
         envShift = 0;  // envelope bins of 2^envShift table rows, at most kEnvBins of them
         while (((I - 1) >> envShift) >= kEnvBins) ++envShift;
+        envShift += num.env_coarsen;
         hcp.assign(n, CondParams());
         hchunks.clear();
         int64_t base = 0;
@@ -539,6 +545,7 @@ public:
         dpr.gasTempEffect = num.gas_temperature_effect;
         dpr.ionizOp = num.ionization_operator;
         dpr.sharingFactor = num.ionization_operator == EKMC_IONIZ_ONE_TAKES_ALL ? 0.0 : 0.5;
+        dpr.parkLanes = num.park_lanes ? num.park_lanes : 32;
         dpr.proc = prec.p;
         dpr.gasFirst = iproc.p; dpr.gasLast = iproc.p + pl.G; dpr.gasFrac = dproc.p;
         // collision-table sources: prepared cross sections and interpolation constants
@@ -1533,6 +1540,8 @@ void ekmc_numerics_default(ekmc_numerics* n) {
     n->max_sync_points = 0;
     n->seed = 0;
     n->work_chunk = 0;
+    n->park_lanes = 0;
+    n->env_coarsen = 0;
     n->reserved = 0;
 }
 
@@ -1553,6 +1562,66 @@ ekmc_status ekmc_create(const ekmc_mixture* mixture, const ekmc_condition* condi
 
 void ekmc_destroy(ekmc_engine* engine) { delete engine; }
 
+// One autotuner trial: a fresh engine of the batch with `nm`, `warmup`
+// untimed and `steps` timed intervals; *rate = collision events per second.
+static ekmc_status autotune_trial(const ekmc_mixture* mixture, const ekmc_condition* conditions, int32_t n_cond,
+                                  ekmc_numerics nm, int32_t warmup, int32_t steps, double* rate) {
+    // no condition may stop inside the trial
+    nm.n_integration_points = 1e300;
+    nm.max_collisions_after_ss = 1e300;
+    nm.max_sync_points = 0;
+    ekmc_engine* raw = nullptr;
+    ekmc_status st = ekmc_create(mixture, conditions, n_cond, &nm, &raw);
+    if (st != EKMC_OK) return st;
+    std::unique_ptr<ekmc_engine> e(raw);
+    int running = 0;
+    if ((st = e->impl->initialize()) != EKMC_OK) return st;
+    if (warmup > 0 && (st = e->impl->step(warmup, &running)) != EKMC_OK) return st;
+    const uint64_t ev0 = e->impl->totalEvents;
+    const auto t0 = Clock::now();
+    if ((st = e->impl->step(steps, &running)) != EKMC_OK) return st;
+    const double seconds = ms_since(t0) * 1e-3;
+    *rate = seconds > 0 ? (double)(e->impl->totalEvents - ev0) / seconds : 0.0;
+    return EKMC_OK;
+}
+
+ekmc_status ekmc_autotune_launch(const ekmc_mixture* mixture, const ekmc_condition* conditions, int32_t n_cond,
+                                 const ekmc_numerics* numerics, int32_t warmup, int32_t steps, ekmc_numerics* best,
+                                 double* best_rate) {
+    if (!mixture || !conditions || !numerics || !best || n_cond <= 0)
+        return fail(EKMC_ERR_INVALID_ARGUMENT, "null argument or no conditions");
+    if (warmup < 0 || steps < 1) return fail(EKMC_ERR_INVALID_ARGUMENT, "autotune needs warmup >= 0, steps >= 1");
+    ekmc_numerics cur = *numerics;
+    cur.park_lanes = 0;
+    cur.env_coarsen = 0;
+    double rate = 0;
+    ekmc_status st = ekmc_autotune(mixture, conditions, n_cond, &cur, nullptr, 0, warmup, steps, &cur.work_chunk,
+                                   nullptr);
+    if (st != EKMC_OK) return st;
+    if ((st = autotune_trial(mixture, conditions, n_cond, cur, warmup, steps, &rate)) != EKMC_OK) return st;
+    // coordinate descent: the lane threshold at the tuned work unit, then the
+    // envelope coarseness at both
+    static const int32_t kPark[] = {16, 24, 40, 48};
+    ekmc_numerics trial = cur;
+    for (int32_t p : kPark) {
+        trial.park_lanes = p;
+        double r = 0;
+        if ((st = autotune_trial(mixture, conditions, n_cond, trial, warmup, steps, &r)) != EKMC_OK) return st;
+        if (r > rate) { rate = r; cur.park_lanes = p; }
+    }
+    trial = cur;
+    for (int32_t ec = 1; ec <= 2; ++ec) {
+        trial.env_coarsen = ec;
+        double r = 0;
+        if ((st = autotune_trial(mixture, conditions, n_cond, trial, warmup, steps, &r)) != EKMC_OK) return st;
+        if (r > rate) { rate = r; cur.env_coarsen = ec; }
+    }
+    if (cur.park_lanes == 0) cur.park_lanes = 32;
+    *best = cur;
+    if (best_rate) *best_rate = rate;
+    return EKMC_OK;
+}
+
 ekmc_status ekmc_autotune(const ekmc_mixture* mixture, const ekmc_condition* conditions, int32_t n_cond,
                           const ekmc_numerics* numerics, const int32_t* candidates, int32_t n_candidates,
                           int32_t warmup, int32_t steps, int32_t* best_work_chunk, double* rates) {
@@ -1572,22 +1641,9 @@ ekmc_status ekmc_autotune(const ekmc_mixture* mixture, const ekmc_condition* con
     for (int32_t i = 0; i < n_candidates; ++i) {
         ekmc_numerics nm = *numerics;
         nm.work_chunk = candidates[i];
-        // no condition may stop inside the trial
-        nm.n_integration_points = 1e300;
-        nm.max_collisions_after_ss = 1e300;
-        nm.max_sync_points = 0;
-        ekmc_engine* raw = nullptr;
-        ekmc_status st = ekmc_create(mixture, conditions, n_cond, &nm, &raw);
+        double rate = 0;
+        ekmc_status st = autotune_trial(mixture, conditions, n_cond, nm, warmup, steps, &rate);
         if (st != EKMC_OK) return st;
-        std::unique_ptr<ekmc_engine> e(raw);
-        int running = 0;
-        if ((st = e->impl->initialize()) != EKMC_OK) return st;
-        if (warmup > 0 && (st = e->impl->step(warmup, &running)) != EKMC_OK) return st;
-        const uint64_t ev0 = e->impl->totalEvents;
-        const auto t0 = Clock::now();
-        if ((st = e->impl->step(steps, &running)) != EKMC_OK) return st;
-        const double seconds = ms_since(t0) * 1e-3;
-        const double rate = seconds > 0 ? (double)(e->impl->totalEvents - ev0) / seconds : 0.0;
         if (rates) rates[i] = rate;
         if (rate > best) {
             best = rate;

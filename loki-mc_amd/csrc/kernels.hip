@@ -51,16 +51,20 @@ static inline void timer_stop(int k, hipStream_t st) {
 constexpr double kFix = 16777216.0;  // 2^24: fixed-point scale of energy sums [eV]
 static inline unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 constexpr int kWavesPerBlock = kBlock / kWave;
-// Lanes of a wave that wait for a real collision before the wave runs the
-// collision operator for all of them.
-constexpr int kParkMin = 32;
+// Null-collision events a lane may run in one phase-A step of the flight
+// kernel: a null event costs a flight and a null test, not a whole step of
+// the wave's bookkeeping (measured per interval on the 64-condition scan:
+// 1 / 2 / 3 / 4 events, 0.981 / 0.963 / 0.989 / 1.062 ms; longer runs leave
+// more lanes idle in the loop than they save).
+template <typename V> constexpr int null_run() { return 2; }
 
 // LDS of one wave of the flight kernel: collision tallies [3P] (64-bit), the
 // histogram grid of the chunk's condition [8] (FP64), the null-test envelope
 // [2 kEnvBins] (V) and the EEDF counts of a histogram taken at load [histLds]
+// (16-bit, two cells per word: a chunk holds at most 65535 main electrons)
 // (32-bit).  The record ring is a separate static array.
 template <typename V> __host__ __device__ constexpr size_t flight_wave_lds(int P, int histLds) {
-    return (size_t)3 * P * 8 + 8 * 8 + (size_t)2 * kEnvBins * sizeof(V) + (size_t)histLds * 4;
+    return (size_t)3 * P * 8 + 8 * 8 + (size_t)2 * kEnvBins * sizeof(V) + (size_t)((histLds + 1) / 2) * 4;
 }
 
 // ------------------------------------------------------------ wave helpers
@@ -396,7 +400,7 @@ __device__ __forceinline__ void hist_count(const DevHist& h, int c, const double
     const double en = (double)e.eps;
     const int cell = (int)(en / g[0]);
     if (cell >= 0 && cell < h.nCells) {
-        if (HIST != HIST_HBM) lds_add_u32(lhist + cell, 1u);
+        if (HIST != HIST_HBM) lds_add_u32(lhist + (cell >> 1), (cell & 1) ? 0x10000u : 1u);
         else {
             atomicAdd(&h.eedf[(size_t)c * h.nCells + cell], 1ull);
             if (row >= 0 && h.eedfPer) atomicAdd(&h.eedfPer[((size_t)c * h.nPhases + row) * h.nCells + cell], 1ull);
@@ -448,7 +452,7 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
     const int histRow = histOn ? h.phaseRow[c] : -1;
     const bool histCyl = HIST != HIST_LDS && histOn && h.eadf && h.cyl[c] != 0;
     if (histLds)
-        for (int i = lane; i < h.nCells; i += kWave) lhist[i] = 0;
+        for (int i = lane; i < (h.nCells + 1) / 2; i += kWave) lhist[i] = 0;
     wave_lds_sync();
 
     const int count = ch.count;
@@ -514,35 +518,44 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
         bool done = false, dead = false, spawn = false;
         CollOut<V> o;
         o.pid = -1;
-        // phase A: flight, and the null test of the cold branch
+        // phase A: flight, and the null test of the cold branch; a lane whose
+        // event is a null collision flies on in the same step, up to null_run<V>()
+        // events (the draws are keyed by the event, so the run length changes
+        // no trajectory)
         if (have && !parked) {
-            {
+            for (int run = 0; run < null_run<V>(); ++run) {
                 double gain = 0;
                 EventRng<V> rng(cp.seed, e.stream, e.event);
                 const int st = step_a<V, FC, THERMAL>(cp, pr, nutot, envHi, envLo, e, rng, o, gain);
                 fieldGain += gain;
                 if (st == STEP_DONE) {
                     done = true;
-                } else if (!isfinite(e.t) || !isfinite((double)e.eps)) {
+                    break;
+                }
+                if (!isfinite(e.t) || !isfinite((double)e.eps)) {
                     status |= ST_NONFINITE;
                     e.t = cp.tsync;
                     done = true;
-                } else {
-                    if (o.escape) ++nEsc;
-                    if (st == STEP_NULL) {
-                        ++nNull;
-                        after_event(cp, e);
-                        if (++iter >= kMaxLoopPerSync) { status |= ST_LOOP_CAP; done = true; }
-                    } else {
-                        parked = true;
-                        c0[0] = rng.w0[0]; c0[1] = rng.w0[1]; c0[2] = rng.w0[2]; c0[3] = rng.w0[3];
-                    }
+                    break;
+                }
+                if (o.escape) ++nEsc;
+                if (st != STEP_NULL) {
+                    parked = true;
+                    c0[0] = rng.w0[0]; c0[1] = rng.w0[1]; c0[2] = rng.w0[2]; c0[3] = rng.w0[3];
+                    break;
+                }
+                ++nNull;
+                after_event(cp, e);
+                if (++iter >= kMaxLoopPerSync) {
+                    status |= ST_LOOP_CAP;
+                    done = true;
+                    break;
                 }
             }
         }
         // phase B: real collisions, once enough lanes wait for one (or all do)
         const uint64_t pk = __ballot(parked);
-        if (pk && (__popcll(pk) >= kParkMin || pk == (live & ~__ballot(done)))) {
+        if (pk && (__popcll(pk) >= pr.parkLanes || pk == (live & ~__ballot(done)))) {
             if (parked) {
                 EventRng<V> rng(cp.seed, e.stream, e.event, c0);
                 step_b<V, THERMAL>(cp, pr, cumul, interp, e, rng, o);
@@ -638,7 +651,7 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
     if (histLds) {
         const int row = h.phaseRow[c];
         for (int i = lane; i < h.nCells; i += kWave) {
-            const uint32_t n = lhist[i];
+            const uint32_t n = (lhist[i >> 1] >> ((i & 1) * 16)) & 0xffffu;
             if (!n) continue;
             atomicAdd(&h.eedf[(size_t)c * h.nCells + i], (unsigned long long)n);
             if (row >= 0 && h.eedfPer)
@@ -648,10 +661,21 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
     wave_lds_sync();
 }
 
+// Waves per SIMD the flight kernel is compiled for.  FP64: 256 registers per
+// lane, two waves; left to itself the compiler takes 290 (256 VGPRs and 34
+// AGPRs), one wave per SIMD, and the few VGPRs spilled at two waves cost less
+// than the latency one wave cannot hide (measured 2.27 -> 1.60 ms per
+// interval on the 64-condition scan in FP64).  Mixed form: the compiler's
+// choice (150-166 VGPRs, three waves, for the FP32 flight formulas); asking
+// for three waves explicitly gave the same occupancy and a slower schedule
+// (0.962 -> 1.000 ms), asking for four (with a two-segment record ring to
+// fit the LDS) spilled 33-41 VGPRs and ran at 1.062 ms.
+template <typename V, int FC> constexpr int flight_waves_per_eu() { return sizeof(V) == 8 ? 2 : 1; }
+
 // Persistent flight kernel of one field case: every wave takes chunks from
 // the list through an atomic counter until the list is exhausted.
 template <typename V, int FC, bool THERMAL, int HIST>
-__global__ __launch_bounds__(kBlock) void k_advance(DevState<V> s, DevWork w, DevProcesses pr, const V* tables,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(flight_waves_per_eu<V, FC>()))) void k_advance(DevState<V> s, DevWork w, DevProcesses pr, const V* tables,
                                                     DevHist h, const int32_t* list, int32_t nList,
                                                     uint32_t* queue) {
     extern __shared__ unsigned char smem[];

@@ -222,12 +222,13 @@ int main(int argc, char** argv) {
         }
         const ekmc_mixture view = mix.view();
         if (autotune) {
-            int32_t best = 0;
-            if (ekmc_autotune(&view, jobs.conds.data(), (int32_t)jobs.conds.size(), &num, nullptr, 0, 100, 100,
-                              &best, nullptr) != EKMC_OK)
+            ekmc_numerics best{};
+            if (ekmc_autotune_launch(&view, jobs.conds.data(), (int32_t)jobs.conds.size(), &num, 100, 100, &best,
+                                     nullptr) != EKMC_OK)
                 fatal(ekmc_last_error());
-            num.work_chunk = best;
-            std::printf("autotuned work unit: %d electrons\n", best);
+            num = best;
+            std::printf("autotuned launch: work unit %d electrons, collision step at %d lanes, envelope coarsening %d\n",
+                        best.work_chunk, best.park_lanes, best.env_coarsen);
         }
         ekmc_engine* eng = nullptr;
         if (ekmc_create(&view, jobs.conds.data(), (int32_t)jobs.conds.size(), &num, &eng) != EKMC_OK) fatal(ekmc_last_error());

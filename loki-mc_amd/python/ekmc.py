@@ -83,7 +83,8 @@ class Numerics(C.Structure):
                 ("n_integrated_ss_times", C.c_double), ("integrated_absolute_time", C.c_double),
                 ("sync_time_x_max_coll_freq", C.c_double), ("sync_over_sampling", C.c_int32),
                 ("max_sync_points", C.c_int32), ("seed", C.c_uint64),
-                ("work_chunk", C.c_int32), ("reserved", C.c_int32)]
+                ("work_chunk", C.c_int32), ("park_lanes", C.c_int32), ("env_coarsen", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class Results(C.Structure):
@@ -371,6 +372,7 @@ def lib():
         L.ekmc_destroy.argtypes = [C.c_void_p]
         L.ekmc_destroy.restype = None
         L.ekmc_autotune.argtypes = [_p(Mixture), _p(Condition), I32, _p(Numerics), pI32, I32, I32, I32, pI32, pD]
+        L.ekmc_autotune_launch.argtypes = [_p(Mixture), _p(Condition), I32, _p(Numerics), I32, I32, _p(Numerics), pD]
         for name in ("ekmc_solve", "ekmc_initialize", "ekmc_finalize", "ekmc_reset_kernel_stats"):
             getattr(L, name).argtypes = [C.c_void_p]
         L.ekmc_step.argtypes = [C.c_void_p, I32, pI32]
@@ -419,6 +421,19 @@ def autotune(mix, conditions, numerics, candidates=None, warmup=100, steps=100):
                               cand.ctypes.data_as(C.POINTER(C.c_int32)), len(cand), warmup, steps,
                               C.byref(best), dptr(rates)))
     return best.value, {int(c): float(r) for c, r in zip(cand, rates)}
+
+
+def autotune_launch(mix, conditions, numerics, warmup=100, steps=100):
+    """ekmc_autotune_launch: work unit, collision-step lane threshold and
+    envelope coarseness of the fastest short run.  Returns (numerics with
+    the three set, events/s)."""
+    conds = list(conditions)
+    arr = (Condition * len(conds))(*conds)
+    best = Numerics()
+    rate = C.c_double()
+    check(lib().ekmc_autotune_launch(C.byref(mix.struct), arr, len(conds), C.byref(numerics), warmup, steps,
+                                     C.byref(best), C.byref(rate)))
+    return best, rate.value
 
 
 class Engine:

@@ -1,6 +1,9 @@
-"""Work-unit autotuner (ekmc_autotune): argument checks on the host, and on
-the GPU a tuning pass over a small batch that returns one of its candidates
-with a measured rate for every candidate."""
+"""Launch autotuners.  ekmc_autotune (work unit): argument checks on the
+host, and on the GPU a tuning pass over a small batch that returns one of its
+candidates with a measured rate for every candidate.  ekmc_autotune_launch
+(work unit, collision-step lane threshold, envelope coarseness): argument
+checks, and on the GPU a pass whose pick runs and gives the same physics as
+the defaults within the statistics."""
 import ctypes as C
 
 import numpy as np
@@ -45,3 +48,51 @@ def test_autotune_picks_a_candidate_and_the_pick_runs():
     assert eng.step(5) == len(conds)
     assert eng.event_count()[0] > 0
     eng.close()
+
+
+def test_launch_parameters_out_of_range_are_rejected():
+    mix, conds, num = ekmc.workload("cfg1_atomic40_dc", n_electrons=100)
+    for field, bad in (("park_lanes", -1), ("park_lanes", 65), ("env_coarsen", -1), ("env_coarsen", 4),
+                       ("work_chunk", 65536)):
+        n = ekmc.Numerics.from_buffer_copy(num)
+        setattr(n, field, bad)
+        with pytest.raises(ekmc.EngineError, match=field):
+            ekmc.Engine(mix, conds, n)
+
+
+def test_launch_autotune_rejects_bad_arguments():
+    mix, conds, num = ekmc.workload("cfg1_atomic40_dc", n_electrons=100)
+    arr = (ekmc.Condition * len(conds))(*conds)
+    best = ekmc.Numerics()
+    best.work_chunk = -7
+    L = ekmc.lib()
+    assert L.ekmc_autotune_launch(C.byref(mix.struct), arr, len(conds), C.byref(num), 0, 0, C.byref(best), None) == 1
+    assert L.ekmc_autotune_launch(C.byref(mix.struct), arr, 0, C.byref(num), 0, 5, C.byref(best), None) == 1
+    assert best.work_chunk == -7
+
+
+@pytest.mark.gpu
+def test_launch_autotune_pick_runs_and_keeps_the_physics():
+    mix, conds, num = ekmc.workload("cfg3_o2like_batched", n_electrons=20000, conditions=[3, 21, 40, 60])
+    num.precision = ekmc.MIXED
+    num.seed = 5
+    best, rate = ekmc.autotune_launch(mix, conds, num, warmup=10, steps=10)
+    print("autotune_launch", best.work_chunk, best.park_lanes, best.env_coarsen, rate)
+    assert rate > 0 and best.work_chunk >= 64
+    assert best.park_lanes in (16, 24, 32, 40, 48) and best.env_coarsen in (0, 1, 2)
+    assert best.seed == num.seed and best.precision == num.precision
+    # every launch setting is a schedule of the same per-electron draws: the
+    # first interval's events are those of the default launch, for the pick
+    # and for settings away from the defaults in every parameter
+    other = ekmc.Numerics.from_buffer_copy(num)
+    other.work_chunk, other.park_lanes, other.env_coarsen = 512, 16, 2
+    counts = []
+    for n in (num, best, other):
+        eng = ekmc.Engine(mix, conds, n)
+        eng.initialize()
+        eng.step(1)
+        counts.append(eng.event_count())
+        eng.close()
+    print(counts)
+    assert counts[0][1] > 0
+    assert counts[1] == counts[0] and counts[2] == counts[0]  # (events, real collisions), bit for bit

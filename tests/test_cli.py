@@ -147,13 +147,16 @@ REID = {1: (0.1014, 0.06), 12: (0.2691, 0.10), 24: (0.4081, 0.10)}
 
 
 @pytest.mark.gpu
-def test_autotuned_run_reports_its_work_unit_and_writes_the_outputs(tmp_path):
+def test_autotuned_run_reports_its_launch_and_writes_the_outputs(tmp_path):
     inp = _input_tree(tmp_path)
     r = _run([str(inp / "ReidRamp_setup_DC.in"), "--input", str(inp), "--output", str(tmp_path / "Output"),
               "--seed", "6", "--precision", "mixed", "--autotune"], tmp_path)
     assert r.returncode == 0, r.stderr
-    m = re.search(r"autotuned work unit: (\d+) electrons", r.stdout)
-    assert m and int(m.group(1)) in (256, 512, 768, 1024, 1536, 2048)
+    m = re.search(r"autotuned launch: work unit (\d+) electrons, collision step at (\d+) lanes, "
+                  r"envelope coarsening (\d+)", r.stdout)
+    assert m, r.stdout
+    assert int(m.group(1)) in (256, 512, 768, 1024, 1536, 2048)
+    assert int(m.group(2)) in (16, 24, 32, 40, 48) and int(m.group(3)) in (0, 1, 2)
     for en in REID:
         sub = tmp_path / "Output" / "ReidRamp_DC" / ("reducedElecField_%g" % en)
         assert (sub / "swarmParameters.txt").exists() and (sub / "eedf.txt").exists()
