@@ -62,8 +62,9 @@ template <typename T> struct DevBuf {
 constexpr size_t kCollideStageBytes = (3 + 1 + 1 + 11 + 3 + 1 + 1) * sizeof(double) + sizeof(int32_t);
 
 // Host-side phases reported after the kernels by ekmc_get_kernel_stats.
-enum HostPhase { H_PREPARE = 0, H_WAIT, H_POST, H_TABLES, H_FLUSH, H_COUNT };
-const char* kHostNames[H_COUNT] = {"host_prepare", "host_wait", "host_post", "host_tables", "host_hist_flush"};
+enum HostPhase { H_PREPARE = 0, H_WAIT, H_POST, H_TABLES, H_FLUSH, H_LAUNCH, H_COUNT };
+const char* kHostNames[H_COUNT] = {"host_prepare", "host_wait", "host_post", "host_tables", "host_hist_flush",
+                                   "host_launch"};
 using Clock = std::chrono::steady_clock;
 inline double ms_since(Clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
@@ -240,7 +241,8 @@ public:
     std::vector<CondParams> hcp;
     std::vector<Chunk> hchunks;
     // device buffers
-    DevBuf<Rec<V>> rec;
+    DevBuf<RecA<V>> recA;  // electron records (FP64: their sampled part)
+    DevBuf<RecB<V>> recB;  // FP64: the draw part of the records
     DevBuf<uint32_t> holeList, markKeys, status, queue;
     DevBuf<uint8_t> alive;
     DevBuf<V> tables;
@@ -295,7 +297,7 @@ public:
             if (timer.start[k]) (void)hipEventDestroy(timer.start[k]);
             if (timer.stop[k]) (void)hipEventDestroy(timer.stop[k]);
         }
-        rec.release(); holeList.release(); markKeys.release(); status.release(); queue.release();
+        recA.release(); recB.release(); holeList.release(); markKeys.release(); status.release(); queue.release();
         alive.release(); tables.release(); dconds.release(); chunks.release(); holesC.release();
         surplusC.release(); markC.release(); flist.release(); phaseRow.release(); cylFlag.release();
         holeOff.release(); surOff.release(); markOff.release(); mainBase.release(); tailC.release();
@@ -482,7 +484,8 @@ public:
         totalMain = sumN;
         if (capacity > 0xFFFFFFF0LL) return fail(EKMC_ERR_CAPACITY, "electron pool exceeds 2^32 slots");
         // ---- device allocations
-        HIP_TRY(rec.alloc(capacity));
+        HIP_TRY(recA.alloc(capacity));
+        if (split_record<V>()) HIP_TRY(recB.alloc(capacity));
         HIP_TRY(alive.alloc(capacity));
         HIP_TRY(holeList.alloc(capacity));
         HIP_TRY(markKeys.alloc(capacity));
@@ -602,7 +605,7 @@ public:
         std::vector<int32_t> cyl(n);
         for (int c = 0; c < n; ++c) cyl[c] = cs[c].cylSym ? 1 : 0;
         HIP_TRY(hipMemcpyAsync(cylFlag.p, cyl.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-        ds = DevState<V>{rec.p, alive.p};
+        ds = DevState<V>{recA.p, recB.p, alive.p};
         dw.conds = dconds.p; dw.chunks = chunks.p; dw.holesC = holesC.p; dw.surplusC = surplusC.p;
         dw.markC = markC.p; dw.holeOff = holeOff.p; dw.surOff = surOff.p; dw.markOff = markOff.p;
         dw.holeList = holeList.p; dw.markKeys = markKeys.p; dw.tailC = tailC.p; dw.momC = momC.p; dw.accCount = accCount.p;
@@ -1125,6 +1128,8 @@ public:
                 ekmc_status st = rebuild_lists();
                 if (st != EKMC_OK) return st;
             }
+            host_add(H_PREPARE, ms_since(tPrep));
+            const auto tLaunch = Clock::now();
             std::memcpy(pinCond.p, hcp.data(), nCond * sizeof(CondParams));
             HIP_TRY(hipMemcpyAsync(dconds.p, pinCond.p, nCond * sizeof(CondParams), hipMemcpyHostToDevice, stream));
             HIP_TRY(launch_flights<V>(ds, dw, dpr, tables.p, dh, fl, stream));
@@ -1134,7 +1139,7 @@ public:
             }
             ekmc_status st = launch_cycle();
             if (st != EKMC_OK) return st;
-            host_add(H_PREPARE, ms_since(tPrep));
+            host_add(H_LAUNCH, ms_since(tLaunch));
             if ((st = wait_summary()) != EKMC_OK) return st;
             const auto tPost = Clock::now();
             for (int c = 0; c < nCond; ++c) {
@@ -1157,19 +1162,27 @@ public:
         const int64_t begin = hcp[c].base, end = c + 1 < nCond ? hcp[c + 1].base : capacity;
         const size_t n = (size_t)(end - begin);
         std::vector<Rec<V>> h(n);
+        std::vector<RecA<V>> ha(n);
+        std::vector<RecB<V>> hb(split_record<V>() ? n : 0);
         std::vector<uint8_t> al(n);
         HIP_TRY(hipStreamSynchronize(stream));
-        HIP_TRY(hipMemcpy(h.data(), rec.p + begin, n * sizeof(Rec<V>), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(ha.data(), recA.p + begin, n * sizeof(RecA<V>), hipMemcpyDeviceToHost));
+        if (split_record<V>())
+            HIP_TRY(hipMemcpy(hb.data(), recB.p + begin, n * sizeof(RecB<V>), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            h[i].a = ha[i];
+            if constexpr (split_record<V>()) h[i].b = hb[i];
+        }
         HIP_TRY(hipMemcpy(al.data(), alive.p + begin, n, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < n; ++i) {
             if (!al[i]) continue;
-            const Rec<V>& q = h[i];
+            const RecA<V>& q = ha[i];
             sn.pos.insert(sn.pos.end(), {q.x, q.y, q.z});
             sn.vel.insert(sn.vel.end(), {(double)q.vx, (double)q.vy, (double)q.vz});
             sn.energy.push_back(host_energy(q.vx, q.vy, q.vz));
             sn.time.push_back(q.t);
-            sn.stream.push_back(q.stream);
-            sn.event.push_back(q.event);
+            sn.stream.push_back(draw_part(h[i]).stream);
+            sn.event.push_back(draw_part(h[i]).event);
         }
         sn.tsync = cs[c].tsync;
         sn.trial = cs[c].trial;
@@ -1188,19 +1201,20 @@ public:
         if (stbits & ST_POPCTL) return fail(EKMC_ERR_CAPACITY, "population control could not draw an exact sample");
         if (stbits & ST_LOOP_CAP) return fail(EKMC_ERR_CAPACITY, "flight cap reached within one interval");
         if (stbits & ST_NONFINITE) return fail(EKMC_ERR_STATE, "an electron time or energy became non-finite");
-        // nonParallelCollisionTasks bookkeeping (BoltzmannMC.C:1291-1339)
+        // nonParallelCollisionTasks bookkeeping (BoltzmannMC.C:1291-1339);
+        // the counts arrive as exact non-negative integers in doubles
         for (int p = 0; p < P; ++p) {
-            C.counts[p] += (uint64_t)llround(ev[p]);
+            C.counts[p] += (uint64_t)ev[p];
             C.gainP[p] += ev[P + p];
             C.lossP[p] += ev[2 * P + p];
         }
         C.fieldGain += ev[3 * P + EV_FIELD];
         C.growth += ev[3 * P + EV_GROWTH];
-        uint64_t real = (uint64_t)llround(ev[3 * P + EV_REAL]), nul = (uint64_t)llround(ev[3 * P + EV_NULL]);
+        uint64_t real = (uint64_t)ev[3 * P + EV_REAL], nul = (uint64_t)ev[3 * P + EV_NULL];
         C.total += real;
         C.nullc += nul;
-        C.events += (uint64_t)llround(ev[3 * P + EV_EVENTS]);
-        C.escapes += (uint64_t)llround(ev[3 * P + EV_ESCAPE]);
+        C.events += (uint64_t)ev[3 * P + EV_EVENTS];
+        C.escapes += (uint64_t)ev[3 * P + EV_ESCAPE];
         totalEvents += real + nul;
         totalReal += real;
         C.lastMaxE = m[26];
@@ -1213,8 +1227,10 @@ public:
                 std::snprintf(buf, sizeof buf, "The energy of the electrons reached %f eV, while at least one of the elastic cross sections are defined only until %f eV.", C.maxElecEnergy, pl.energyMaxElastic);
                 return fail(EKMC_ERR_ENERGY_OUT_OF_TABLE, buf);
             }
-            int dec = (int)std::fmax(std::log(C.nSamp) / std::log(2.0) - 11, 6);
-            int nSS = (int)std::pow(2, dec);
+            // the reference's cadence expressions, evaluated only where they
+            // can exceed their floor (below 2^17 samples dec is 6)
+            int dec = C.nSamp < (1 << 17) ? 6 : (int)std::fmax(std::log(C.nSamp) / std::log(2.0) - 11, 6);
+            int nSS = 1 << dec;
             if (C.ssTime != kNonDef) {
                 ++C.nIntPts;
                 C.afterSS = C.total - C.atSS;
@@ -1230,8 +1246,8 @@ public:
                     C.histPhase = C.omega != 0 ? phase_index(C) : -1;
                 }
             }
-            dec = (int)std::fmax(std::log(C.nIntPts) / std::log(2.0) - 9, 7);
-            int nErr = (int)std::pow(2, dec);
+            dec = C.nIntPts < (1 << 16) ? 7 : (int)std::fmax(std::log(C.nIntPts) / std::log(2.0) - 9, 7);
+            int nErr = 1 << dec;
             if (C.ssTime != kNonDef && C.nIntPts > 200 && C.nIntPts % nErr == 0) check_statistical_errors(C);
         }
         bool stop = !keep_running(C);
@@ -1288,11 +1304,11 @@ public:
     ekmc_status get_electrons(int c, double* pos, double* vel, double* e, double* tt) override {
         const CondParams& cp = hcp[c];
         const int64_t n = cp.nMain;
-        std::vector<Rec<V>> h(n);
+        std::vector<RecA<V>> h(n);
         HIP_TRY(hipStreamSynchronize(stream));
-        HIP_TRY(hipMemcpy(h.data(), rec.p + cp.base, n * sizeof(Rec<V>), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(h.data(), recA.p + cp.base, n * sizeof(RecA<V>), hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < n; ++i) {
-            const Rec<V>& q = h[i];
+            const RecA<V>& q = h[i];
             if (pos) { pos[3 * i] = q.x; pos[3 * i + 1] = q.y; pos[3 * i + 2] = q.z; }
             if (vel) { vel[3 * i] = q.vx; vel[3 * i + 1] = q.vy; vel[3 * i + 2] = q.vz; }
             if (e) e[i] = host_energy(q.vx, q.vy, q.vz);

@@ -7,30 +7,65 @@
 
 namespace ekmc {
 
-// One electron in HBM, one record per pool slot (array of records, so that a
-// lane taking a new electron reads one contiguous 64- or 96-byte record).
-// The energy is not stored: it is 0.5 m_e |v|^2 / e of the stored velocity,
-// which is what the last flight of every electron computed.
-template <typename V> struct Rec;
-template <> struct __attribute__((aligned(16))) Rec<float> {
+// One electron in HBM.  Mixed form: one 64-byte record per pool slot, a
+// whole HBM sector, so that storing an electron never writes part of a
+// sector.  FP64 form: two records per slot in two arrays, RecA (64 bytes:
+// position, clock, velocity, pending free time: the state an ensemble sample
+// reads) and RecB (32 bytes: trial frequency of the free time, event counter,
+// Philox stream), so that the sampling kernel reads 64 of the 96 bytes.  (The
+// same split of the mixed record, 48 + 16 bytes, made the sampling kernel 34
+// us faster per interval on the 64-condition scan and the flight kernel 46 us
+// slower: a 48-byte record shares sectors with its neighbours.)  The flight
+// kernel's ring loads both parts, piece by piece, into the layout of Rec,
+// which is RecA followed by RecB.  The energy is not stored: it is 0.5 m_e
+// |v|^2 / e of the stored velocity, which is what the last flight of every
+// electron computed.
+template <typename V> constexpr bool split_record() { return sizeof(V) == 8; }
+template <typename V> struct RecA;
+template <> struct __attribute__((aligned(16))) RecA<float> {
     double x, y, z, t;          // position [m], clock [s]
     float vx, vy, vz, tfree;    // velocity [m/s], pending free time [s] (< 0: none)
     float nue;                  // trial frequency its free time was drawn with
     uint32_t event;             // collision counter (Philox counter word)
     uint64_t stream;            // Philox stream of this electron
 };
-template <> struct __attribute__((aligned(16))) Rec<double> {
+template <> struct __attribute__((aligned(16))) RecA<double> {
     double x, y, z, t;
     double vx, vy, vz, tfree;
+};
+template <typename V> struct RecB;
+template <> struct __attribute__((aligned(16))) RecB<float> {};  // not used: the mixed record is whole
+template <> struct __attribute__((aligned(16))) RecB<double> {
     double nue;
     uint32_t event, pad0;
     uint64_t stream, pad1;
 };
-static_assert(sizeof(Rec<float>) == 64, "mixed record is 64 bytes");
-static_assert(sizeof(Rec<double>) == 96, "FP64 record is 96 bytes");
+template <typename V> struct Rec;
+template <> struct Rec<float> {
+    RecA<float> a;
+};
+template <> struct Rec<double> {
+    RecA<double> a;
+    RecB<double> b;
+};
+static_assert(sizeof(RecA<float>) == 64 && sizeof(Rec<float>) == 64, "mixed record is 64 bytes");
+static_assert(sizeof(RecA<double>) == 64 && sizeof(RecB<double>) == 32 && sizeof(Rec<double>) == 96,
+              "FP64 record is 64 + 32 bytes");
+
+// The part of a record that holds its trial frequency, event counter and
+// stream.
+template <typename V> __host__ __device__ inline const auto& draw_part(const Rec<V>& q) {
+    if constexpr (split_record<V>()) return q.b;
+    else return q.a;
+}
+template <typename V> __host__ __device__ inline auto& draw_part(Rec<V>& q) {
+    if constexpr (split_record<V>()) return q.b;
+    else return q.a;
+}
 
 template <typename V> struct DevState {
-    Rec<V>* rec;      // [pool capacity]
+    RecA<V>* ra;      // [pool capacity]
+    RecB<V>* rb;      // [pool capacity] FP64 form only
     uint8_t* alive;   // [pool capacity] 0 dead/empty, 1 alive, 2 clone made this interval
 };
 
@@ -159,11 +194,11 @@ hipError_t launch_finish(const DevWork& w, int P, hipStream_t st);
 // (histogram configuration h: the EEDF in LDS when h.histLds >= h.nCells)
 template <typename V>
 int flight_blocks_per_cu(int fc, bool thermal, int P, const DevHist& h);
-// Record ring of the flight kernel: segments of kRingSeg main records per
-// wave, ring_segs<V>() of them (the mixed record is smaller, so its ring is
-// deeper in the same LDS).
+// Record ring of the flight kernel: ring_segs() segments of kRingSeg main
+// records per wave (8 KB per wave for the mixed record, 12 KB for FP64; both
+// still fit the blocks per CU their registers allow).
 constexpr int kRingSeg = 32;
-template <typename V> constexpr int ring_segs() { return sizeof(V) == 4 ? 3 : 2; }
+template <typename V> constexpr int ring_segs() { return 4; }
 // Static LDS of one flight-kernel block: the record rings of its waves.
 template <typename V> constexpr size_t flight_ring_lds() {
     return (size_t)ring_segs<V>() * (kBlock / kWave) * kRingSeg * sizeof(Rec<V>);

@@ -53,10 +53,11 @@ static inline unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) /
 constexpr int kWavesPerBlock = kBlock / kWave;
 // Null-collision events a lane may run in one phase-A step of the flight
 // kernel: a null event costs a flight and a null test, not a whole step of
-// the wave's bookkeeping (measured per interval on the 64-condition scan:
-// 1 / 2 / 3 / 4 events, 0.981 / 0.963 / 0.989 / 1.062 ms; longer runs leave
-// more lanes idle in the loop than they save).
-template <typename V> constexpr int null_run() { return 2; }
+// the wave's bookkeeping.  Measured per interval on the 64-condition scan,
+// mixed form: 1 / 2 / 3 / 4 events, 0.981 / 0.963 / 0.989 / 1.062 ms (longer
+// runs leave more lanes idle in the loop than they save); FP64 form, whose
+// second flight spills registers: 1 / 2 events, 1.596 / 1.663 ms.
+template <typename V> constexpr int null_run() { return sizeof(V) == 8 ? 1 : 2; }
 
 // LDS of one wave of the flight kernel: collision tallies [3P] (64-bit), the
 // histogram grid of the chunk's condition [8] (FP64), the null-test envelope
@@ -111,21 +112,34 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // ------------------------------------------------------------ records
-template <typename V> __device__ __forceinline__ void load_rec(const Rec<V>* r, int64_t i, Electron<V>& e) {
-    const Rec<V> q = r[i];
-    e.x = q.x; e.y = q.y; e.z = q.z; e.t = q.t;
-    e.vx = q.vx; e.vy = q.vy; e.vz = q.vz;
-    e.eps = energy_of(q.vx, q.vy, q.vz);
-    e.tfree = q.tfree; e.nue = q.nue; e.stream = q.stream; e.event = q.event;
+template <typename V> __device__ __forceinline__ void rec_to_electron(const Rec<V>& q, Electron<V>& e) {
+    e.x = q.a.x; e.y = q.a.y; e.z = q.a.z; e.t = q.a.t;
+    e.vx = q.a.vx; e.vy = q.a.vy; e.vz = q.a.vz;
+    e.eps = energy_of(q.a.vx, q.a.vy, q.a.vz);
+    e.tfree = q.a.tfree;
+    e.nue = draw_part(q).nue; e.stream = draw_part(q).stream; e.event = draw_part(q).event;
 }
-template <typename V> __device__ __forceinline__ void store_rec(Rec<V>* r, int64_t i, const Electron<V>& e) {
+template <typename V> __device__ __forceinline__ Rec<V> rec_get(const DevState<V>& s, int64_t i) {
+    Rec<V> q;
+    q.a = s.ra[i];
+    if constexpr (split_record<V>()) q.b = s.rb[i];
+    return q;
+}
+template <typename V> __device__ __forceinline__ void rec_put(const DevState<V>& s, int64_t i, const Rec<V>& q) {
+    s.ra[i] = q.a;
+    if constexpr (split_record<V>()) s.rb[i] = q.b;
+}
+template <typename V> __device__ __forceinline__ void load_rec(const DevState<V>& s, int64_t i, Electron<V>& e) {
+    rec_to_electron(rec_get(s, i), e);
+}
+template <typename V> __device__ __forceinline__ void store_rec(const DevState<V>& s, int64_t i, const Electron<V>& e) {
     Rec<V> q = {};
-    q.x = e.x; q.y = e.y; q.z = e.z; q.t = e.t;
-    q.vx = e.vx; q.vy = e.vy; q.vz = e.vz; q.tfree = e.tfree;
-    q.nue = e.nue; q.event = e.event; q.stream = e.stream;
-    r[i] = q;
+    q.a.x = e.x; q.a.y = e.y; q.a.z = e.z; q.a.t = e.t;
+    q.a.vx = e.vx; q.a.vy = e.vy; q.a.vz = e.vz; q.a.tfree = e.tfree;
+    draw_part(q).nue = e.nue; draw_part(q).event = e.event; draw_part(q).stream = e.stream;
+    rec_put(s, i, q);
 }
-template <typename V> __device__ __forceinline__ double rec_energy(const Rec<V>& q) {
+template <typename V> __device__ __forceinline__ double rec_energy(const RecA<V>& q) {
     return (double)energy_of(q.vx, q.vy, q.vz);
 }
 
@@ -147,6 +161,7 @@ template <typename V> __device__ __forceinline__ double rec_energy(const Rec<V>&
 // segment cannot overtake it.
 static_assert(kRingSeg == kWave / 2, "one ring slot per lane of the lower half");
 template <typename V> __host__ __device__ constexpr int rec_pieces() { return (int)(sizeof(Rec<V>) / 16); }
+template <typename V> __host__ __device__ constexpr int rec_pieces_a() { return (int)(sizeof(RecA<V>) / 16); }
 static_assert(flight_ring_lds<float>() == (size_t)ring_segs<float>() * kWavesPerBlock * kRingSeg * sizeof(Rec<float>),
               "ring size");
 typedef __attribute__((address_space(1))) void global_void;
@@ -159,15 +174,18 @@ template <typename V> __device__ __forceinline__ lds_uint4* ring_segment(lds_uin
 // Loads of main records [loaded, loaded + kRingSeg) of a chunk of `count`
 // into their segment.
 template <typename V>
-__device__ __forceinline__ void ring_fill(const Rec<V>* rec, int64_t first, int count, int loaded, lds_uint4* ring) {
+__device__ __forceinline__ void ring_fill(const DevState<V>& s, int64_t first, int count, int loaded, lds_uint4* ring) {
     const int lane = lane_id();
     const int idx = loaded + lane;
     lds_uint4* seg = ring_segment<V>(ring, loaded);
     if (lane < kRingSeg && idx < count) {
-        const char* src = (const char*)(rec + first + idx);
+        const char* srcA = (const char*)(s.ra + first + idx);
+        const char* srcB = (const char*)(s.rb + first + idx);
 #pragma unroll
-        for (int k = 0; k < rec_pieces<V>(); ++k)
-            __builtin_amdgcn_global_load_lds((global_void*)(src + 16 * k), (lds_void*)(seg + k * kRingSeg), 16, 0, 0);
+        for (int k = 0; k < rec_pieces<V>(); ++k) {
+            const char* src = k < rec_pieces_a<V>() ? srcA + 16 * k : srcB + 16 * (k - rec_pieces_a<V>());
+            __builtin_amdgcn_global_load_lds((global_void*)src, (lds_void*)(seg + k * kRingSeg), 16, 0, 0);
+        }
     }
 }
 // s_waitcnt with vmcnt(N) (expcnt and lgkmcnt not waited for)
@@ -223,10 +241,7 @@ template <typename V> __device__ __forceinline__ void ring_read(const lds_uint4*
     ring_pieces((uint32_t)(size_t)(seg + j), p);
     Rec<V> q;
     __builtin_memcpy(&q, p, sizeof(q));
-    e.x = q.x; e.y = q.y; e.z = q.z; e.t = q.t;
-    e.vx = q.vx; e.vy = q.vy; e.vz = q.vz;
-    e.eps = energy_of(q.vx, q.vy, q.vz);
-    e.tfree = q.tfree; e.nue = q.nue; e.stream = q.stream; e.event = q.event;
+    rec_to_electron(q, e);
 }
 
 // ------------------------------------------------------- block helpers
@@ -335,7 +350,7 @@ __global__ __launch_bounds__(kBlock) void k_init(DevState<V> s, const CondParams
     e.nue = (V)cp.trial;
     e.stream = (uint64_t)i;
     e.event = 0;
-    store_rec(s.rec, slot, e);
+    store_rec(s, slot, e);
     s.alive[slot] = 1;
 }
 
@@ -362,8 +377,8 @@ __device__ void wave_refill(const DevState<V>& s, uint32_t* holeList, const Chun
             else if (r - holes != j) dst = ch.region + (r - holes);
         }
         if (dst >= 0) {
-            const Rec<V> q = s.rec[ch.region + j];
-            s.rec[dst] = q;
+            const Rec<V> q = rec_get(s, ch.region + j);
+            rec_put(s, dst, q);
             s.alive[ch.region + j] = 0;
         }
         __builtin_amdgcn_wave_barrier();
@@ -461,7 +476,7 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
     int landed = 0;  // ring fills known to have landed
     constexpr int kAhead = (ring_segs<V>() - 1) * kRingSeg;
     for (; loaded < count && loaded <= nextMain + kAhead; loaded += kRingSeg)
-        ring_fill<V>(s.rec, ch.first, count, loaded, ring);
+        ring_fill<V>(s, ch.first, count, loaded, ring);
     Electron<V> e;
     bool have = false, isNew = false, parked = false;
     int64_t slot = 0;
@@ -500,7 +515,7 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
                 }
                 if (have) {
                     if (isNew) {
-                        load_rec(s.rec, slot, e);
+                        load_rec(s, slot, e);
                     } else {
                         const int idx = (int)(slot - ch.first);
                         ring_read<V>(ring_segment<V>(ring, idx), idx % kRingSeg, e);
@@ -589,7 +604,7 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
                     n.tfree = (V)kNonDef; n.nue = (V)cp.trial;
                     n.stream = child_stream(cp.seed, e.stream, e.event - 1);
                     n.event = 0;
-                    store_rec(s.rec, ch.region + idx, n);
+                    store_rec(s, ch.region + idx, n);
                     s.alive[ch.region + idx] = 1;
                 } else {
                     status |= ST_REGION_OVERFLOW;
@@ -605,16 +620,23 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
                 s.alive[slot] = 0;
                 if (!isNew) w.holeList[ch.first + holes + lane_rank(dm)] = (uint32_t)slot;
             } else {
-                store_rec(s.rec, slot, e);
+                store_rec(s, slot, e);
                 if (!isNew) s.alive[slot] = 1;  // clears the clone mark of the last interval
                 maxE = fmax(maxE, (double)e.eps);
             }
             have = false;
         }
         holes += __popcll(dm);
-        // refill consumed ring segments
-        for (; loaded < count && loaded <= nextMain + kAhead; loaded += kRingSeg)
-            ring_fill<V>(s.rec, ch.first, count, loaded, ring);
+        // refill consumed ring segments, two or more at a time: every wait
+        // for a table load also waits for the ring fills issued before it (a
+        // wave's vector memory operations complete in order), so fewer, larger
+        // fills leave fewer steps waiting on HBM latency (flight kernel per
+        // interval on the 64-condition scan, mixed / FP64: 3 and 2 segments
+        // filled one at a time 0.964 / 1.613 ms, 4 segments two at a time
+        // 0.936 / 1.526 ms)
+        if (loaded + kRingSeg <= nextMain + kAhead)
+            for (; loaded < count && loaded <= nextMain + kAhead; loaded += kRingSeg)
+                ring_fill<V>(s, ch.first, count, loaded, ring);
     }
 
     int remaining = 0;
@@ -945,7 +967,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_marks(DevState<V> s, DevWork w
         }
         const uint64_t km = __ballot(killMain);
         if (kill) {
-            growth -= rec_energy(s.rec[slot]);
+            growth -= rec_energy(s.ra[slot]);
             s.alive[slot] = 0;
             if (killMain) w.holeList[ch.first + holes + lane_rank(km)] = (uint32_t)slot;
         }
@@ -1000,7 +1022,7 @@ __global__ __launch_bounds__(kBlock) void k_move(DevState<V> s, DevWork w) {
         const int64_t ts = owner_of(soff, cp.nChunks, i);
         const Chunk& sc = w.chunks[cp.chunkBegin + ts];
         const int64_t src = sc.region + (i - soff[ts]);
-        s.rec[dst] = s.rec[src];
+        rec_put(s, dst, rec_get(s, src));
         s.alive[src] = 0;
         s.alive[dst] = 1;
     }
@@ -1036,14 +1058,14 @@ __global__ __launch_bounds__(kBlock) void k_clone(DevState<V> s, DevWork w) {
             if (s.alive[cp.base + cand] == 1) { src = cp.base + cand; break; }
         }
         if (src < 0) { status |= ST_POPCTL; continue; }
-        Rec<V> q = s.rec[src];
-        growth += rec_energy(q);
+        Rec<V> q = rec_get(s, src);
+        growth += rec_energy(q.a);
         uint32_t wds[4];
-        philox_call(cp.seed ^ 0x5851F42D4C957F2Dull, q.stream, (uint32_t)dst, P_CLONE,
+        philox_call(cp.seed ^ 0x5851F42D4C957F2Dull, draw_part(q).stream, (uint32_t)dst, P_CLONE,
                     (uint32_t)(cp.syncIndex & 0xffffu), wds);
-        q.stream = ((uint64_t)wds[1] << 32) | wds[0];
-        q.event = 0;
-        s.rec[dst] = q;
+        draw_part(q).stream = ((uint64_t)wds[1] << 32) | wds[0];
+        draw_part(q).event = 0;
+        rec_put(s, dst, q);
         s.alive[dst] = 2;  // alive, but not a clone source in this pass (BoltzmannMC.C:1374)
     }
     const uint32_t st = wave_or(status);
@@ -1067,7 +1089,7 @@ __global__ __launch_bounds__(kBlock) void k_sample(DevState<V> s, DevWork w) {
     if (!w.conds[ch.cond].active) return;
     double m[kMomentRecord];
     for (int k = 0; k < kMomentRecord; ++k) m[k] = 0;
-    auto add = [&](const Rec<V>& q) {
+    auto add = [&](const RecA<V>& q) {
         const double r[3] = {q.x, q.y, q.z};
         const double v[3] = {(double)q.vx, (double)q.vy, (double)q.vz};
         const double e = rec_energy(q);
@@ -1082,11 +1104,11 @@ __global__ __launch_bounds__(kBlock) void k_sample(DevState<V> s, DevWork w) {
     // slot order
     int64_t k = lane;
     for (; k + kWave < ch.count; k += 2 * kWave) {
-        const Rec<V> q0 = s.rec[ch.first + k], q1 = s.rec[ch.first + k + kWave];
+        const RecA<V> q0 = s.ra[ch.first + k], q1 = s.ra[ch.first + k + kWave];
         add(q0);
         add(q1);
     }
-    if (k < ch.count) add(s.rec[ch.first + k]);
+    if (k < ch.count) add(s.ra[ch.first + k]);
     for (int k = 0; k < kMomentRecord - 1; ++k) m[k] = wave_sum(m[k]);
     m[26] = wave_max(m[26]);
     if (lane == 0)
@@ -1115,7 +1137,7 @@ __global__ __launch_bounds__(kBlock) void k_hist(DevState<V> s, const SlotRange*
     const bool cyl = h.cyl[c] != 0 && h.eadf;
     const int row = h.phaseRow[c];
     for (int64_t k = threadIdx.x; k < rg.count; k += kBlock) {
-        const Rec<V> q = s.rec[rg.first + k];
+        const RecA<V> q = s.ra[rg.first + k];
         const double e = rec_energy(q);
         const int cell = (int)(e / eStep);
         if (cell >= 0 && cell < h.nCells) {
