@@ -244,6 +244,11 @@ def main():
     ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed",
                     help="arithmetic of the batched run (fp64: the same batch in the plain form's arithmetic, "
                          "for profiling the FP64 flight kernel)")
+    ap.add_argument("--swarm-only", action="store_true",
+                    help="print only the mixed-vs-FP64 swarm comparison (sizes below) as one JSON line")
+    ap.add_argument("--swarm-electrons", type=int, default=20000)
+    ap.add_argument("--swarm-points", type=int, default=1500)
+    ap.add_argument("--swarm-replicas", type=int, default=8)
     args = ap.parse_args()
     if args.plain_steps is None:
         args.plain_steps = args.steps
@@ -259,6 +264,9 @@ def main():
               "final gather of per-condition results)." % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.swarm_only:
+        print(json.dumps(swarm_err(args.swarm_electrons, args.swarm_points, args.swarm_replicas)))
+        return
     if args.kernel_breakdown:
         os.environ["EKMC_KERNEL_TIMING"] = "all"
     mix, conds, num = ekmc.workload(WORKLOAD)
@@ -352,7 +360,7 @@ def main():
         out["speedup_vs_plain_fp64"] = value / (pe / ps)
         print("bench: plain fp64 %.3e events/s" % (pe / ps), file=sys.stderr, flush=True)
     if not args.no_swarm:
-        out["swarm_err"] = swarm_err()
+        out["swarm_err"] = swarm_err(args.swarm_electrons, args.swarm_points, args.swarm_replicas)
         print("bench: swarm parameters mixed vs fp64: max %.2f sigma, median %.2f sigma, %s" %
               (out["swarm_err"]["max_sigma"], out["swarm_err"]["median_sigma"], out["swarm_err"]["verdicts"]),
               file=sys.stderr, flush=True)

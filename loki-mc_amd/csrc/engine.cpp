@@ -409,10 +409,12 @@ public:
             ldsMax)
             return fail(EKMC_ERR_INVALID_ARGUMENT, "too many processes for the per-wave collision tallies in LDS");
         // Default work unit: 1024 main electrons per wave where that still
-        // gives every resident wave of the flight kernel (3 blocks of 4 waves
-        // per CU) a chunk, else the batch split over those waves, at least
-        // 256 (per-chunk work and newborn regions grow as chunks shrink).
-        // On one MI355X a batch of more than 3.1 M electrons keeps 1024; the
+        // gives 12 waves per CU a chunk (the mixed flight kernel's residency,
+        // 3 blocks of 4 waves), else the batch split over that many waves,
+        // at least 256 (per-chunk work and newborn regions grow as chunks
+        // shrink).  The FP64 kernel holds 8 waves per CU, so it gets about
+        // 1.5 chunks per wave (its engines were tuned with this rule).  On one
+        // MI355X a batch of more than 3.1 M electrons keeps 1024; the
         // one-condition engines of the 64-condition scan (1e4-1e6 electrons)
         // ran 24 % faster with 256 than with 1024.
         int64_t chunkTarget = num.work_chunk;
