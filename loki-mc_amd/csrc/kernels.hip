@@ -529,6 +529,17 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
         }
         const uint64_t live = __ballot(have);
         if (live == 0) break;
+        // refill the ring segments this step's lanes emptied, two or more at a
+        // time, before the flights: every wait for a table load also waits for
+        // the ring fills issued before it (a wave's vector memory operations
+        // complete in order), so fewer, larger and earlier fills leave fewer
+        // steps waiting on HBM latency (flight kernel per interval on the
+        // 64-condition scan, mixed / FP64: 3 and 2 segments filled one at a
+        // time at the end of a step 0.964 / 1.613 ms, 4 segments two at a time
+        // 0.936 / 1.526 ms, the same filled before the flights 0.920 ms mixed)
+        if (loaded + kRingSeg <= nextMain + kAhead)
+            for (; loaded < count && loaded <= nextMain + kAhead; loaded += kRingSeg)
+                ring_fill<V>(s, ch.first, count, loaded, ring);
 
         bool done = false, dead = false, spawn = false;
         CollOut<V> o;
@@ -627,16 +638,6 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
             have = false;
         }
         holes += __popcll(dm);
-        // refill consumed ring segments, two or more at a time: every wait
-        // for a table load also waits for the ring fills issued before it (a
-        // wave's vector memory operations complete in order), so fewer, larger
-        // fills leave fewer steps waiting on HBM latency (flight kernel per
-        // interval on the 64-condition scan, mixed / FP64: 3 and 2 segments
-        // filled one at a time 0.964 / 1.613 ms, 4 segments two at a time
-        // 0.936 / 1.526 ms)
-        if (loaded + kRingSeg <= nextMain + kAhead)
-            for (; loaded < count && loaded <= nextMain + kAhead; loaded += kRingSeg)
-                ring_fill<V>(s, ch.first, count, loaded, ring);
     }
 
     int remaining = 0;
