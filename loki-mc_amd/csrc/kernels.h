@@ -66,7 +66,8 @@ template <typename V> __host__ __device__ inline auto& draw_part(Rec<V>& q) {
 template <typename V> struct DevState {
     RecA<V>* ra;      // [pool capacity]
     RecB<V>* rb;      // [pool capacity] FP64 form only
-    uint8_t* alive;   // [pool capacity] 0 dead/empty, 1 alive, 2 clone made this interval
+    uint8_t* alive;   // [pool capacity] 0 dead/empty, 1 alive, 2 clone made by this interval's
+                      // population control (back to 1 in the sampling kernel)
 };
 
 // A wave's unit of work: up to `count` main electrons of one condition plus
