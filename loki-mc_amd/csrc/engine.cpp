@@ -106,6 +106,7 @@ inline double host_energy(float vx, float vy, float vz) {
 
 // Per-condition host state: the loop variables of BoltzmannMC.
 struct Cond {
+    double workPerElectron = 1.0;  // collision events per main electron in the last interval
     ekmc_condition in;
     double N = 0, Tg = 0, gasEnergy = 0;
     double E[3] = {0, 0, 0}, accel[3] = {0, 0, 0}, omega = 0, cyc = 0;
@@ -256,6 +257,7 @@ public:
     DevBuf<PopPlan> plans;
     FlightLists fl = {};
     std::vector<int32_t> listActive;  // active flags the flight lists were built for
+    int listAge = 0;                  // intervals since the flight lists were ordered
     // histogram grid and phase rows as last uploaded (staged in pinned memory)
     PinnedBuf<double> pinGrid;
     PinnedBuf<int32_t> pinRow;
@@ -749,7 +751,10 @@ public:
     ekmc_status rebuild_lists() {
         std::vector<int32_t> act(nCond);
         for (int c = 0; c < nCond; ++c) act[c] = cs[c].finished ? 0 : 1;
-        if (act == listActive) return EKMC_OK;
+        // rebuilt when the running set changes, and every 256 intervals for
+        // the order (below)
+        if (act == listActive && ++listAge < 256) return EKMC_OK;
+        listAge = 0;
         listActive = act;
         hlist.clear();
         fl = FlightLists{};
@@ -770,11 +775,16 @@ public:
                 if (cls != fc) continue;
                 for (int j = 0; j < hcp[c].nChunks; ++j) hlist.push_back(hcp[c].chunkBegin + j);
             }
-            // larger chunks first, so that the last chunks taken are the
-            // short ones and the waves finish together (the processing order
-            // does not change any result)
-            std::stable_sort(hlist.begin() + begin, hlist.end(),
-                             [&](int32_t a, int32_t b) { return hchunks[a].count > hchunks[b].count; });
+            // more work first (main electrons times the condition's events per
+            // electron in its last interval), so that the last chunks taken
+            // are the short ones and the waves finish together; the
+            // processing order does not change any result (flight kernel per
+            // interval on the 64-condition scan: 0.896 ms ordered by size,
+            // 0.890 ms by expected work)
+            std::stable_sort(hlist.begin() + begin, hlist.end(), [&](int32_t a, int32_t b) {
+                return hchunks[a].count * cs[hchunks[a].cond].workPerElectron >
+                       hchunks[b].count * cs[hchunks[b].cond].workPerElectron;
+            });
             const int32_t count = (int32_t)(hlist.size() - begin);
             fl.list[fc] = flist.p + begin;
             fl.count[fc] = count;
@@ -1216,6 +1226,7 @@ public:
         C.total += real;
         C.nullc += nul;
         C.events += (uint64_t)ev[3 * P + EV_EVENTS];
+        C.workPerElectron = C.in.n_electrons > 0 ? ev[3 * P + EV_EVENTS] / (double)C.in.n_electrons : 1.0;
         C.escapes += (uint64_t)ev[3 * P + EV_ESCAPE];
         totalEvents += real + nul;
         totalReal += real;
