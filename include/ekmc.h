@@ -367,7 +367,8 @@ const char* ekmc_last_error(void);
  * Philox4x32-10 with counter {S lo, S hi, k, call} and key = seed:
  *   FP64: 53 bits of words (0,1) or (2,3); call 0 {0,5}, 1 {6,7}, 2 {1,2},
  *         3 {3,4}, 4 {8,9}, 5 {10}
- *   FP32: top 24 bits of one word; call 0 {0,5,6,7}, 1 {1,2,3,4}, 2 {8,9,10} */
+ *   FP32: the same calls, top 24 bits of word 0 or 2: the 24 leading bits
+ *         of the FP64 variate */
 #define EKMC_VARIATES_PER_EVENT 11
 
 /* Collision tables of one condition (interpolateCrossSections,

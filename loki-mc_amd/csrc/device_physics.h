@@ -47,25 +47,23 @@ __device__ __forceinline__ constexpr int slot_kind(int s) {
     return s == 0 ? 0 : ((s == 6 || s == 8 || s == 9) ? 2 : 1);
 }
 
-// Slot -> Philox call of the event.  Call 0 carries everything a cold
-// collision event needs (free time, channel choice, cos(chi), azimuth), so a
-// flight that ends in a collision costs one call; thermal and ionization
-// slots come from later calls.
+// Slot -> Philox call of the event: call 0 carries the free time and the
+// channel-choice variate, so a null event costs one call; a cold collision
+// draws cos(chi) and the azimuth from call 1; thermal and ionization slots
+// come from later calls.
 //   FP64 (one slot = 53 bits of two words, two slots per call):
 //     call 0 {0,5}  1 {6,7}  2 {1,2}  3 {3,4}  4 {8,9}  5 {10}
-//   FP32 (one slot = the top 24 bits of one word, four slots per call):
-//     call 0 {0,5,6,7}  1 {1,2,3,4}  2 {8,9,10}
+//   FP32: the same calls; a slot is the top 24 bits of the first of its two
+//     words, i.e. the 24 leading bits of the FP64 variate.  With the same
+//     seed the two forms then draw the same variates, and their trajectories
+//     stay together until a rounding difference flips a decision.  (Packing
+//     four FP32 slots into call 0 saved the second call of a cold collision,
+//     about 1 % of the flight kernel, but left the two forms uncorrelated.)
 __device__ __forceinline__ constexpr int call64(int s) {
     return (s == 0 || s == 5) ? 0 : (s == 6 || s == 7) ? 1 : (s <= 2) ? 2 : (s <= 4) ? 3 : (s <= 9) ? 4 : 5;
 }
 __device__ __forceinline__ constexpr int half64(int s) {
     return (s == 5 || s == 7 || s == 2 || s == 4 || s == 9) ? 1 : 0;
-}
-__device__ __forceinline__ constexpr int call32(int s) {
-    return (s == 0 || (s >= 5 && s <= 7)) ? 0 : (s <= 4 ? 1 : 2);
-}
-__device__ __forceinline__ constexpr int word32(int s) {
-    return s == 0 ? 0 : (s <= 4 ? s - 1 : (s <= 7 ? s - 4 : s - 8));
 }
 
 // Variates of one event: call 0 cached in w0, one other call in w1.  With a
@@ -108,9 +106,8 @@ template <> __device__ __forceinline__ double EventRng<double>::u(int s) {
 
 template <> __device__ __forceinline__ float EventRng<float>::u(int s) {
     if (tape) return (float)tape[s];
-    const uint32_t* w = words(call32(s));
-    const int q = word32(s);
-    const uint32_t v = q == 0 ? w[0] : q == 1 ? w[1] : q == 2 ? w[2] : w[3];
+    const uint32_t* w = words(call64(s));
+    const uint32_t v = half64(s) ? w[2] : w[0];
     const float x = (float)(v >> 8);
     const float k = 1.0f / 16777216.0f;
     const int kind = slot_kind(s);

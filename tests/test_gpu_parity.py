@@ -545,6 +545,41 @@ def test_mixed_precision_matches_fp64_statistically():
             assert abs(z) <= 3.0, (c, name, z)
 
 
+PAIRED_NAMES = REPLICA_NAMES[:7]
+
+
+def test_mixed_precision_follows_fp64_with_the_same_variates():
+    """The mixed form draws the FP64 form's variates (their 24 leading bits),
+    so with the same seed the two forms follow the same trajectories until a
+    rounding difference flips a decision, and on a gas without ionization or
+    attachment (no population control) their results stay paired.  On the
+    reference's Reid-ramp input at 12 and 24 Td, 16 replicas of 2e5
+    electrons per form: for mean energy, flux and bulk drift and flux and
+    bulk transverse and longitudinal diffusion, the replicas' mean relative
+    difference plus three of its standard errors stays within 1e-4 (the
+    unpaired replica spread is 2e-3 - 3e-2)."""
+    from published import reid_mixture
+    R = 16
+    tds = (12.0, 24.0)
+    conds = [condition(en=en, n_e=200000) for en in tds for _ in range(R)]
+    res = {}
+    for prec in (ekmc.FP64, ekmc.MIXED):
+        num = _short_numerics(31, prec)
+        num.n_integration_points = 1500
+        eng = _engine(reid_mixture(), conds, num)
+        eng.solve()
+        res[prec] = np.array([_replica_swarm(eng.results(c))[:7] for c in range(len(conds))]).reshape(
+            len(tds), R, -1)
+        eng.close()
+    for ti, en in enumerate(tds):
+        a, b = res[ekmc.FP64][ti], res[ekmc.MIXED][ti]
+        for k, name in enumerate(PAIRED_NAMES):
+            rel = (b[:, k] - a[:, k]) / abs(a[:, k].mean())
+            dev, se = rel.mean(), rel.std(ddof=1) / np.sqrt(R)
+            print(en, name, "paired rel. difference %.2e +- %.2e" % (dev, se))
+            assert abs(dev) + 3 * se <= 1e-4, (en, name, dev, se)
+
+
 def test_reid_ramp_matches_published_loki_mc_values():
     """Whole loop on the reference's own Reid-ramp input (its LXCat file and
     setup: 4 amu, 133.32 Pa, 300 K, E/N = 1, 12, 24 Td batched in one FP64
