@@ -308,6 +308,59 @@ def test_interval_with_newborns_matches_oracle_pool(case):
     eng.close()
 
 
+@pytest.mark.parametrize("case", ["dc", "dcb", "ionizing_dc", "ionizing_dcb"])
+def test_one_interval_of_the_mixed_form_follows_the_oracle(case):
+    """The mixed form draws the FP64 form's variates to 24 bits, so one
+    interval of it from the same start follows the FP64 oracle's passes
+    electron by electron (a rounding difference may flip a decision for at
+    most 1e-4 of them): the same event count for every electron, the same
+    real collisions per process and, in the ionizing gas, the same ejected
+    electrons (child streams); velocities, energies and positions agree to
+    FP32 accuracy, 99.9 % within 3e-6 relative and all within 1e-4."""
+    n = 65536
+    if case in ("dc", "dcb"):
+        mix = _no_ionization_gas()
+        cond = condition(en=120.0, n_e=n) if case == "dc" else condition(en=120.0, angle=90.0, bn=300.0, n_e=n)
+        num = ekmc.numerics_default()
+        num.initial_elec_temp_over_gas_temp = 38.68
+        num.seed = 1234
+    else:
+        mix, _, _ = ekmc.workload("cfg1_atomic40_dc")
+        cond = condition(en=400.0, n_e=n) if case == "ionizing_dc" else condition(en=400.0, angle=90.0, bn=600.0, n_e=n)
+        num = ekmc.numerics_default()
+        num.initial_elec_temp_over_gas_temp = 386.8
+        num.ionization_operator = ekmc.IONIZ["randomUniform"]
+        num.seed = 4321
+    num.precision = ekmc.MIXED
+    eng = _engine(mix, [cond], num)
+    eng.initialize()
+    pos0, vel0, e0, t0 = eng.electrons(0)
+    pool, tsync, trial = eng.op_interval_pool(0)
+    st = eng.status(0)
+    m = O.Model(mix, cond, num)
+    m.tables(st.table_max_energy)
+    state = dict(pos=pos0, vel=vel0, energy=e0, time=t0, free_time=np.full(n, ekmc.NON_DEF, float),
+                 nu_e=np.full(n, trial), stream=np.arange(n, dtype=np.uint64), event=np.zeros(n, np.uint32))
+    opool, counters, nnull = m.advance_pool(num.seed, trial, tsync, state, 4 * n)
+    keep = opool["alive"] == 1
+    o = {k: v[keep] for k, v in opool.items()}
+    _, gi, oi = np.intersect1d(pool["stream"], o["stream"], return_indices=True)
+    same = pool["event"][gi] == o["event"][oi]
+    flipped = max(len(pool["stream"]), len(o["stream"])) - same.sum()
+    print(case, "pool", len(pool["stream"]), "oracle", len(o["stream"]), "not paired", flipped)
+    assert flipped <= 1e-4 * n
+    _, counts, _, _ = eng.process_data(0)
+    assert np.abs(counts.astype(np.int64) - counters.astype(np.int64)).sum() <= 1e-4 * n
+    gi, oi = gi[same], oi[same]
+    for key in ("vel", "pos", "energy"):
+        a, b = pool[key][gi], o[key][oi]
+        rel = (np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1) if a.ndim == 2
+               else np.abs(a - b) / np.abs(b))
+        print("   %s relative difference: 99.9 %% %.2e, max %.2e" % (key, np.quantile(rel, 0.999), rel.max()))
+        assert np.quantile(rel, 0.999) <= 3e-6 and rel.max() <= 1e-4, key
+    eng.close()
+
+
 def _short_numerics(seed, precision=ekmc.FP64):
     num = ekmc.numerics_default()
     num.precision = precision
