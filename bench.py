@@ -195,6 +195,62 @@ def swarm_err(electrons=20000, points=1500, replicas=8, bound=1e-4):
                        "errors of the difference (replica spread), independent seeds"}
 
 
+REID_LXCAT = os.path.join(ROOT, "tests", "golden", "reid_ramp_LXCat.txt")
+PAIRED_TD = (12.0, 24.0)
+PAIRED_NAMES = SWARM_NAMES[:7]
+
+
+def swarm_paired(electrons=200000, points=1500, replicas=16, bound=1e-4):
+    """Mixed precision against FP64 with the same variates: the mixed form
+    draws the FP64 form's variates to 24 bits, so with the same seed the two
+    forms follow the same trajectories until a rounding difference flips a
+    decision.  On the reference's Reid-ramp input (4 amu, an elastic and a
+    0.2 eV inelastic ramp, no ionization or attachment, so no population
+    control pulls the two runs apart) at 12 and 24 Td, `replicas` copies per
+    form: per parameter the mean over replicas of the paired relative
+    difference, its standard error, and the same verdicts as the unpaired
+    comparison."""
+    mix = ekmc.lxcat_mixture(REID_LXCAT, 4.0)
+    p, T = 133.32, 300.0
+    conds = [ekmc.Condition(reduced_elec_field=en, elec_field_angle=180.0, excitation_frequency=0.0,
+                            reduced_mag_field=0.0, gas_temperature=T, gas_density=p / (ekmc.BOLTZMANN * T),
+                            n_electrons=electrons) for en in PAIRED_TD for _ in range(replicas)]
+    res = []
+    for prec in (ekmc.FP64, ekmc.MIXED):
+        num = ekmc.numerics_default()
+        num.precision = prec
+        num.seed = 31
+        num.initial_elec_temp_over_gas_temp = 38.6817
+        num.max_collisions_before_ss = 30
+        num.n_integration_points = points
+        eng = ekmc.Engine(mix, conds, num)
+        eng.solve()
+        res.append(np.array([swarm_parameters(eng.results(c))[:7] for c in range(len(conds))]).reshape(
+            len(PAIRED_TD), replicas, -1))
+        eng.close()
+    rows = []
+    counts = {"within_bound": 0, "unresolved": 0, "deviates": 0}
+    for i, en in enumerate(PAIRED_TD):
+        for k, name in enumerate(PAIRED_NAMES):
+            a, b = res[0][i, :, k], res[1][i, :, k]
+            rel = (b - a) / abs(a.mean())
+            dev, se = float(rel.mean()), float(rel.std(ddof=1) / np.sqrt(replicas))
+            z = dev / se if se > 0 else 0.0
+            unpaired = float(np.hypot(a.std(ddof=1), b.std(ddof=1)) / np.sqrt(replicas) / abs(a.mean()))
+            verdict = ("deviates" if abs(z) > 3 else
+                       "within_bound" if abs(dev) + 3 * se <= bound else "unresolved")
+            counts[verdict] += 1
+            rows.append({"reduced_field_Td": en, "parameter": name, "rel_dev": dev, "rel_se": se,
+                         "rel_se_unpaired": unpaired, "z": z, "verdict": verdict})
+    return {"gas": "Reid ramp (the reference's LXCat input), no population control",
+            "reduced_fields_Td": list(PAIRED_TD), "replicas": replicas, "electrons": electrons,
+            "integration_points": points, "bound": bound, "comparisons": len(rows), "verdicts": counts,
+            "max_abs_dev_plus_3se": max(abs(r["rel_dev"]) + 3 * r["rel_se"] for r in rows),
+            "parameters": rows,
+            "measure": "mixed - fp64 with the same seed, per replica, relative to the fp64 replica mean; "
+                       "mean and standard error over replicas"}
+
+
 def calibrate_copy(mbytes, reps=4):
     """Device-to-device copies of a known size (the runtime's blit kernel), so
     that HBM counters of a profile can be scaled by what they read for a
@@ -245,7 +301,7 @@ def main():
                     help="arithmetic of the batched run (fp64: the same batch in the plain form's arithmetic, "
                          "for profiling the FP64 flight kernel)")
     ap.add_argument("--swarm-only", action="store_true",
-                    help="print only the mixed-vs-FP64 swarm comparison (sizes below) as one JSON line")
+                    help="print only the mixed-vs-FP64 swarm comparisons (the unpaired one at the sizes below, and the paired one) as one JSON line")
     ap.add_argument("--swarm-electrons", type=int, default=20000)
     ap.add_argument("--swarm-points", type=int, default=1500)
     ap.add_argument("--swarm-replicas", type=int, default=8)
@@ -265,7 +321,8 @@ def main():
         sys.exit(2)
 
     if args.swarm_only:
-        print(json.dumps(swarm_err(args.swarm_electrons, args.swarm_points, args.swarm_replicas)))
+        print(json.dumps({"swarm_err": swarm_err(args.swarm_electrons, args.swarm_points, args.swarm_replicas),
+                          "swarm_paired": swarm_paired()}))
         return
     if args.kernel_breakdown:
         os.environ["EKMC_KERNEL_TIMING"] = "all"
@@ -363,6 +420,10 @@ def main():
         out["swarm_err"] = swarm_err(args.swarm_electrons, args.swarm_points, args.swarm_replicas)
         print("bench: swarm parameters mixed vs fp64: max %.2f sigma, median %.2f sigma, %s" %
               (out["swarm_err"]["max_sigma"], out["swarm_err"]["median_sigma"], out["swarm_err"]["verdicts"]),
+              file=sys.stderr, flush=True)
+        out["swarm_paired"] = swarm_paired()
+        print("bench: paired mixed vs fp64 on the Reid ramp: %s, max |dev| + 3 se %.2e" %
+              (out["swarm_paired"]["verdicts"], out["swarm_paired"]["max_abs_dev_plus_3se"]),
               file=sys.stderr, flush=True)
     if not args.no_cpu:
         v, threads, secs_cpu = cpu_baseline(mix, conds, num, args.cpu_intervals, args.cpu_cap)

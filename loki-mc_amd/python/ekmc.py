@@ -191,47 +191,31 @@ def read_lxcat(path):
     return entries
 
 
-def build_mixture(gas_sets, fractions):
-    """Flattened process arrays for a mixture of synthetic gas sets.
-
-    ``gas_sets`` names files ``<set>_LXCat.txt`` under ../gases (each may hold
-    several gases, listed in gases.json); ``fractions`` gives one fraction per
-    gas in order.  Targets are ground states with population 1, so each
-    process's relative density is its gas fraction (State.h:72-85)."""
-    meta = json.load(open(os.path.join(GASES, "gases.json")))
-    cols = {k: [] for k in ("type", "is_elastic", "is_superelastic", "gas_index", "threshold",
+def _new_columns():
+    return {k: [] for k in ("type", "is_elastic", "is_superelastic", "gas_index", "threshold",
                             "target_mass", "rel_density", "stat_weight_ratio", "w_parameter",
                             "scattering", "scattering_params")}
-    offsets, energies, sigmas, names = [0], [], [], []
-    g = 0
-    for set_name in gas_sets:
-        entries = read_lxcat(os.path.join(GASES, "%s_LXCat.txt" % set_name))
-        k = 0
-        for gas in meta[set_name]:
-            procs = gas["processes"]
-            for pm in procs:
-                ent = entries[k]
-                k += 1
-                if ent["type"] == "Effective":
-                    continue
-                ptype = {"Ionization": IONIZATION, "Attachment": ATTACHMENT}.get(ent["type"], CONSERVATIVE)
-                cols["type"].append(ptype)
-                cols["is_elastic"].append(1 if ent["type"] == "Elastic" else 0)
-                cols["is_superelastic"].append(0)
-                cols["gas_index"].append(g)
-                cols["threshold"].append(ent["threshold"])
-                cols["target_mass"].append(gas["mass_amu"] * AMU)
-                cols["rel_density"].append(fractions[g])
-                cols["stat_weight_ratio"].append(0.0)
-                cols["w_parameter"].append(ent["threshold"] if ptype == IONIZATION else 0.0)
-                cols["scattering"].append(SCATTERING[pm["scattering"]])
-                prm = list(pm["params"]) + [0.0, 0.0]
-                cols["scattering_params"] += prm[:2]
-                energies.append(ent["energy"])
-                sigmas.append(ent["sigma"])
-                offsets.append(offsets[-1] + len(ent["energy"]))
-                names.append("%s %s %.4g eV" % (gas["name"], ent["type"], ent["threshold"]))
-            g += 1
+
+
+def _add_process(cols, ent, g, mass_amu, fraction, scattering="isotropic", params=()):
+    """One LXCat entry as a process of gas `g` (ground-state target)."""
+    ptype = {"Ionization": IONIZATION, "Attachment": ATTACHMENT}.get(ent["type"], CONSERVATIVE)
+    cols["type"].append(ptype)
+    cols["is_elastic"].append(1 if ent["type"] == "Elastic" else 0)
+    cols["is_superelastic"].append(0)
+    cols["gas_index"].append(g)
+    cols["threshold"].append(ent["threshold"])
+    cols["target_mass"].append(mass_amu * AMU)
+    cols["rel_density"].append(fraction)
+    cols["stat_weight_ratio"].append(0.0)
+    cols["w_parameter"].append(ent["threshold"] if ptype == IONIZATION else 0.0)
+    cols["scattering"].append(SCATTERING[scattering])
+    prm = list(params) + [0.0, 0.0]
+    cols["scattering_params"] += prm[:2]
+
+
+def _mixture(cols, energies, sigmas, fractions, n_gases, names):
+    offsets = np.concatenate([[0], np.cumsum([len(e) for e in energies])]).astype(np.int32)
     arrays = {
         "type": np.array(cols["type"], np.int32),
         "is_elastic": np.array(cols["is_elastic"], np.int32),
@@ -244,12 +228,57 @@ def build_mixture(gas_sets, fractions):
         "w_parameter": np.array(cols["w_parameter"], np.float64),
         "scattering": np.array(cols["scattering"], np.int32),
         "scattering_params": np.array(cols["scattering_params"], np.float64),
-        "table_offset": np.array(offsets, np.int32),
+        "table_offset": offsets,
         "table_energy": np.concatenate(energies).astype(np.float64),
         "table_sigma": np.concatenate(sigmas).astype(np.float64),
-        "gas_fraction": np.array(fractions[:g], np.float64),
+        "gas_fraction": np.array(fractions[:n_gases], np.float64),
     }
-    return MixtureData(arrays, g, names)
+    return MixtureData(arrays, n_gases, names)
+
+
+def build_mixture(gas_sets, fractions):
+    """Flattened process arrays for a mixture of synthetic gas sets.
+
+    ``gas_sets`` names files ``<set>_LXCat.txt`` under ../gases (each may hold
+    several gases, listed in gases.json); ``fractions`` gives one fraction per
+    gas in order.  Targets are ground states with population 1, so each
+    process's relative density is its gas fraction (State.h:72-85)."""
+    meta = json.load(open(os.path.join(GASES, "gases.json")))
+    cols = _new_columns()
+    energies, sigmas, names = [], [], []
+    g = 0
+    for set_name in gas_sets:
+        entries = read_lxcat(os.path.join(GASES, "%s_LXCat.txt" % set_name))
+        k = 0
+        for gas in meta[set_name]:
+            procs = gas["processes"]
+            for pm in procs:
+                ent = entries[k]
+                k += 1
+                if ent["type"] == "Effective":
+                    continue
+                _add_process(cols, ent, g, gas["mass_amu"], fractions[g], pm["scattering"], pm["params"])
+                energies.append(ent["energy"])
+                sigmas.append(ent["sigma"])
+                names.append("%s %s %.4g eV" % (gas["name"], ent["type"], ent["threshold"]))
+            g += 1
+    return _mixture(cols, energies, sigmas, fractions, g, names)
+
+
+def lxcat_mixture(path, mass_amu):
+    """One gas from an LXCat file: every Elastic, Excitation, Ionization and
+    Attachment entry (Effective ones skipped) as an isotropically scattering
+    process of a ground-state target of `mass_amu`, population 1."""
+    cols = _new_columns()
+    energies, sigmas, names = [], [], []
+    for ent in read_lxcat(path):
+        if ent["type"] == "Effective":
+            continue
+        _add_process(cols, ent, 0, mass_amu, 1.0)
+        energies.append(ent["energy"])
+        sigmas.append(ent["sigma"])
+        names.append("%s %.4g eV" % (ent["type"], ent["threshold"]))
+    return _mixture(cols, energies, sigmas, [1.0], 1, names)
 
 
 def _ptr(a, ctype):
