@@ -76,17 +76,19 @@ def run_engine(mix, conds, num, warmup, steps, preroll_cap=4096):
         eng.step(warmup)
     eng.reset_kernel_stats()
     ev0, real0 = eng.event_count()
+    sim0 = [eng.status(c).time for c in range(len(conds))]
     t0 = time.perf_counter()
     running = eng.step(steps)
     t1 = time.perf_counter()
     ev1, real1 = eng.event_count()
+    simulated = sum(eng.status(c).time - sim0[c] for c in range(len(conds)))
     if running != len(conds):
         raise RuntimeError("%d of %d conditions stopped inside the timed region"
                            % (len(conds) - running, len(conds)))
     stats = eng.kernel_stats()
     escapes = sum(eng.status(c).escape_events for c in range(len(conds)))
     eng.close()
-    return ev1 - ev0, real1 - real0, t1 - t0, stats, preroll, escapes
+    return ev1 - ev0, real1 - real0, t1 - t0, stats, preroll, escapes, simulated
 
 
 def plain_fp64(mix, conds, base_num, warmup, steps, plain_chunk=0):
@@ -102,7 +104,7 @@ def plain_fp64(mix, conds, base_num, warmup, steps, plain_chunk=0):
         num.work_chunk = plain_chunk  # not the batch's tuned launch
         num.park_lanes = 0
         num.env_coarsen = 0
-        ev, re, sec, st, _, esc = run_engine(mix, [c], num, warmup, steps)
+        ev, re, sec, st, _, esc, _ = run_engine(mix, [c], num, warmup, steps)
         events += ev
         real += re
         seconds += sec
@@ -358,7 +360,7 @@ def main():
     print("bench: %s, %d conditions, %d electrons, %d processes" %
           (WORKLOAD, len(conds), n_total, mix.n_processes), file=sys.stderr, flush=True)
 
-    events, real, secs, stats, preroll, escapes = run_engine(mix, conds, num, args.warmup, args.steps)
+    events, real, secs, stats, preroll, escapes, simulated = run_engine(mix, conds, num, args.warmup, args.steps)
     value = events / secs
     ms_per_step = 1e3 * secs / args.steps
     print("bench: batched mixed %.3e events/s (%d events in %.3f s)" % (value, events, secs),
@@ -398,6 +400,9 @@ def main():
                    "work_chunk": num.work_chunk or 1024, "park_lanes": num.park_lanes or 32,
                    "env_coarsen": num.env_coarsen},
         "real_fraction": real / events,
+        "real_events_per_s": real / secs,
+        # simulated time advanced per wall-clock second, summed over the conditions
+        "simulated_seconds_per_s": simulated / secs,
         "escape_events": escapes,
         "events_per_electron_interval": events / (n_total * args.steps),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
