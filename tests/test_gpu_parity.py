@@ -308,7 +308,7 @@ def test_interval_with_newborns_matches_oracle_pool(case):
     eng.close()
 
 
-@pytest.mark.parametrize("case", ["dc", "dcb", "ionizing_dc", "ionizing_dcb"])
+@pytest.mark.parametrize("case", ["dc", "dcb", "ac", "acb", "ecr", "thermal", "ionizing_dc", "ionizing_dcb"])
 def test_one_interval_of_the_mixed_form_follows_the_oracle(case):
     """The mixed form draws the FP64 form's variates to 24 bits, so one
     interval of it from the same start follows the FP64 oracle's passes
@@ -316,13 +316,25 @@ def test_one_interval_of_the_mixed_form_follows_the_oracle(case):
     most 1e-4 of them): the same event count for every electron, the same
     real collisions per process and, in the ionizing gas, the same ejected
     electrons (child streams); velocities, energies and positions agree to
-    FP32 accuracy, 99.9 % within 3e-6 relative and all within 1e-4."""
+    FP32 accuracy, 99.9 % within 3e-6 relative and all within 1e-4.  Every
+    field case (DC, DC+B, AC, AC+B, ECR) and the thermal-target branch."""
     n = 65536
-    if case in ("dc", "dcb"):
+    if not case.startswith("ionizing"):
         mix = _no_ionization_gas()
-        cond = condition(en=120.0, n_e=n) if case == "dc" else condition(en=120.0, angle=90.0, bn=300.0, n_e=n)
+        N = 3.22e22
+        W = ekmc.CHARGE * 800e-27 * N / ekmc.EMASS
+        cond = {"dc": condition(en=120.0, n_e=n),
+                "dcb": condition(en=120.0, angle=90.0, bn=300.0, n_e=n),
+                "ac": condition(en=120.0, freq=3e9, n_e=n),
+                "acb": condition(en=120.0, angle=45.0, freq=2e9, bn=800.0, n_e=n),
+                "ecr": condition(en=120.0, angle=45.0, freq=W / (2 * np.pi), bn=800.0, n_e=n),
+                "thermal": condition(en=5.0, tg=1000.0, n_e=n)}[case]
+        if case in ("ac", "acb", "ecr"):
+            cond.gas_density = N
         num = ekmc.numerics_default()
-        num.initial_elec_temp_over_gas_temp = 38.68
+        num.initial_elec_temp_over_gas_temp = 3.0 if case == "thermal" else 38.68
+        if case == "thermal":
+            num.gas_temperature_effect = ekmc.GASTEMP["true"]
         num.seed = 1234
     else:
         mix, _, _ = ekmc.workload("cfg1_atomic40_dc")
