@@ -171,7 +171,13 @@ typedef struct {
                                                 frequency, staged in LDS, uses bins of
                                                 2^env_coarsen times the fewest rows that fit
                                                 256 bins (0-3; 0: finest) */
-    int32_t reserved;                        /* must be 0 */
+    int32_t evdf3d_cells;                    /* 0: none; 1-256: also count the full 3D velocity
+                                                distribution of every condition from its steady
+                                                state on, on evdf3d_cells^3 cells over
+                                                [-v_max, v_max]^3 per axis (v_max: the speed
+                                                at the top of the EEDF grid, as the 2D EVDF).
+                                                An extension: LoKI-MC has the 2D EVDF of
+                                                cylindrically symmetric cases only */
 } ekmc_numerics;
 
 /* Time-averaged results of one condition (BoltzmannMC.C:1484-1741,1950-2266),
@@ -294,6 +300,14 @@ ekmc_status ekmc_get_eedf_periodic(const ekmc_engine* engine, int32_t cond, doub
 ekmc_status ekmc_get_swarm_periodic(const ekmc_engine* engine, int32_t cond, double* phases, double* mean_energy,
                                     double* flux_velocity, double* bulk_velocity, double* flux_diffusion,
                                     double* bulk_diffusion);
+/* With evdf3d_cells > 0: the time-averaged 3D EVDF of a condition,
+ * [evdf3d_cells^3] row-major over (v_x, v_y, v_z) in the lab frame, normalised
+ * so that its sum times the cell volume is 1 (velocities outside the cube are
+ * not counted), and the cell centres of one axis [evdf3d_cells] in m/s.
+ * Integer counts per sample, so the sums are exact.  No counterpart in the
+ * reference (its EVDF is the 2D one of cylindrically symmetric cases,
+ * BoltzmannMC.C:1562-1571).  Either pointer may be NULL. */
+ekmc_status ekmc_get_evdf3d(const ekmc_engine* engine, int32_t cond, double* axis_cells, double* evdf3d);
 /* Cylindrically symmetric cases only: EVDF [n_radial * n_axial] row-major with
  * radial/axial cell centres, and the first/second anisotropies per energy cell
  * (BoltzmannMC.C:1583-1605). */

@@ -126,6 +126,8 @@ struct Cond {
     // distributions
     double maxEedfE = 0, eedfStep = 0, cosStep = 0, radStep = 0, axStep = 0, maxSpeed = 0;
     std::vector<double> eedfNodes, eedfCells, eehSum, eehPer, eahSum, evhSum, cosCells, radCells, axCells;
+    double v3Step = 0;                   // 3D EVDF cell width (evdf3d_cells > 0)
+    std::vector<double> v3Cells, ev3Sum;  // its cell centres per axis and counts
     int histRequest = 0;   // 1: histogram the current state before the next interval
     int histPhase = -1;
     int nPhases = 0;
@@ -268,13 +270,14 @@ public:
     DevBuf<ProcRec> prec;
     DevBuf<int> iproc;
     DevBuf<double> dproc;
-    DevBuf<unsigned long long> hEedf, hPer, hEadf, hEvdf;
+    DevBuf<unsigned long long> hEedf, hPer, hEadf, hEvdf, hEvdf3;
     DevProcesses dpr;
     DevState<V> ds;
     DevWork dw;
     DevHist dh;
     int summarySize = 0, recSize = 0;
     int nCells = 1000, nCos = 100, nRad = 200, nAx = 200, nPhases = 0;
+    int n3 = 0;  // cells per axis of the 3D EVDF (0: none)
     bool anyCyl = false, anyAC = false;
     PinnedBuf<double> hsum;         // per-condition summaries of the last interval
     PinnedBuf<CondParams> pinCond;  // staging of the per-interval condition upload
@@ -305,7 +308,7 @@ public:
         holeOff.release(); surOff.release(); markOff.release(); mainBase.release(); tailC.release();
         momC.release(); summary.release(); thermalDev.release(); grid.release(); accCount.release();
         accGain.release(); accLoss.release(); plans.release(); prec.release(); iproc.release(); dproc.release();
-        hEedf.release(); hPer.release(); hEadf.release(); hEvdf.release(); ranges.release();
+        hEedf.release(); hPer.release(); hEadf.release(); hEvdf.release(); hEvdf3.release(); ranges.release();
         hsum.release(); pinCond.release(); pinNumax.release(); tsrcD.release(); tsrcI.release();
         pinGrid.release(); pinRow.release();
         nutot64.release(); numax64.release();
@@ -331,7 +334,10 @@ public:
         if (num.park_lanes < 0 || num.park_lanes > kWave)
             return fail(EKMC_ERR_INVALID_ARGUMENT, "park_lanes must be 0 (default) or 1-64");
         if (num.env_coarsen < 0 || num.env_coarsen > 3) return fail(EKMC_ERR_INVALID_ARGUMENT, "env_coarsen must be 0-3");
-        if (num.work_chunk < 0 || num.reserved != 0) return fail(EKMC_ERR_INVALID_ARGUMENT, "invalid work_chunk/reserved");
+        if (num.work_chunk < 0) return fail(EKMC_ERR_INVALID_ARGUMENT, "work_chunk must not be negative");
+        if (num.evdf3d_cells < 0 || num.evdf3d_cells > 256)
+            return fail(EKMC_ERR_INVALID_ARGUMENT, "evdf3d_cells must be 0-256");
+        n3 = num.evdf3d_cells;
         // the flight kernel's LDS EEDF counts a chunk's main electrons in 16 bits
         if (num.work_chunk > 65535) return fail(EKMC_ERR_INVALID_ARGUMENT, "work_chunk must be at most 65535");
         int dev = 0;
@@ -599,9 +605,10 @@ public:
         HIP_TRY(hEedf.alloc((size_t)n * nCells));
         if (anyAC) HIP_TRY(hPer.alloc((size_t)n * nPhases * nCells));
         if (anyCyl) { HIP_TRY(hEadf.alloc((size_t)n * nCells * nCos)); HIP_TRY(hEvdf.alloc((size_t)n * nRad * nAx)); }
+        if (n3 > 0) HIP_TRY(hEvdf3.alloc((size_t)n * n3 * n3 * n3));
         // the histogram sums accumulate from zero (device memory of an earlier
         // engine may be handed out again with its contents)
-        for (DevBuf<unsigned long long>* b : {&hEedf, &hPer, &hEadf, &hEvdf})
+        for (DevBuf<unsigned long long>* b : {&hEedf, &hPer, &hEadf, &hEvdf, &hEvdf3})
             if (b->p) HIP_TRY(hipMemsetAsync(b->p, 0, b->n * sizeof(unsigned long long), stream));
         HIP_TRY(grid.alloc((size_t)n * 6));
         HIP_TRY(phaseRow.alloc(n));
@@ -617,6 +624,7 @@ public:
         dw.status = status.p; dw.queue = queue.p; dw.summary = summary.p; dw.summarySize = summarySize;
         dw.recSize = recSize; dw.nChunks = nChunks; dw.nCond = n;
         dh.eedf = hEedf.p; dh.eedfPer = hPer.p; dh.eadf = hEadf.p; dh.evdf = hEvdf.p; dh.grid = grid.p;
+        dh.evdf3 = hEvdf3.p; dh.n3 = n3;
         dh.phaseRow = phaseRow.p; dh.cyl = cylFlag.p;
         dh.nCells = nCells; dh.nCos = nCos; dh.nRad = nRad; dh.nAx = nAx; dh.nPhases = nPhases;
         // the EEDF of a histogram taken at load is counted in LDS when that
@@ -634,6 +642,8 @@ public:
         // load with HBM atomics contends on the few busy EEDF cells (measured
         // 3.5x slower flights on the 64-condition scan)
         if (dh.histLds < nCells) fusedHist = false;
+        // the 3D EVDF is counted by the histogram kernel only
+        if (n3 > 0) fusedHist = false;
         // first main electron of each condition, for the init kernel
         std::vector<int64_t> mb(n);
         std::vector<double> td(n);
@@ -912,6 +922,10 @@ public:
             if ((st = pull(hEadf, (size_t)c * nCells * nCos, (size_t)nCells * nCos, C.eahSum)) != EKMC_OK) return st;
             if ((st = pull(hEvdf, (size_t)c * nRad * nAx, (size_t)nRad * nAx, C.evhSum)) != EKMC_OK) return st;
         }
+        if (n3 > 0 && hEvdf3.p) {
+            const size_t cube = (size_t)n3 * n3 * n3;
+            if ((st = pull(hEvdf3, (size_t)c * cube, cube, C.ev3Sum)) != EKMC_OK) return st;
+        }
         return EKMC_OK;
     }
 
@@ -982,6 +996,12 @@ public:
         C.axCells = linspaced(nAx, -C.maxSpeed + C.axStep / 2.0, C.maxSpeed - C.axStep / 2.0);
         if (C.cylSym) { C.eahSum.assign((size_t)nCells * nCos, 0.0); C.evhSum.assign((size_t)nRad * nAx, 0.0); }
         if (C.omega != 0) C.eehPer.assign((size_t)nPhases * nCells, 0.0);
+        if (n3 > 0) {
+            // the 3D EVDF grid, like the 2D one, is fixed at the steady state
+            C.v3Step = 2.0 * C.maxSpeed / n3;
+            C.v3Cells = linspaced(n3, -C.maxSpeed + C.v3Step / 2.0, C.maxSpeed - C.v3Step / 2.0);
+            C.ev3Sum.assign((size_t)n3 * n3 * n3, 0.0);
+        }
     }
 
     void check_steady_state(Cond& C) {
@@ -1571,7 +1591,7 @@ void ekmc_numerics_default(ekmc_numerics* n) {
     n->work_chunk = 0;
     n->park_lanes = 0;
     n->env_coarsen = 0;
-    n->reserved = 0;
+    n->evdf3d_cells = 0;
 }
 
 ekmc_status ekmc_create(const ekmc_mixture* mixture, const ekmc_condition* conditions, int32_t n_cond,
@@ -1783,6 +1803,23 @@ ekmc_status ekmc_get_swarm_periodic(const ekmc_engine* e, int32_t cond, double* 
             if (bulkD) bulkD[9 * ph + a] = C.perBulkD[9 * ph + a] / n;
         }
     }
+    return EKMC_OK;
+}
+
+ekmc_status ekmc_get_evdf3d(const ekmc_engine* e, int32_t cond, double* axis, double* evdf3d) {
+    if (!e || cond < 0 || cond >= (int)e->impl->cs.size()) return fail(EKMC_ERR_INVALID_ARGUMENT, "bad argument");
+    const int n3 = e->impl->num.evdf3d_cells;
+    if (n3 == 0) return fail(EKMC_ERR_STATE, "the 3D EVDF was not requested (evdf3d_cells is 0)");
+    if (!e->impl->finalized) return fail(EKMC_ERR_STATE, "distributions are available after ekmc_finalize");
+    const Cond& C = e->impl->cs[cond];
+    if (C.v3Cells.size() != (size_t)n3 || C.ev3Sum.size() != (size_t)n3 * n3 * n3)
+        return fail(EKMC_ERR_STATE, "the condition has no sample after its steady state");
+    double tot = 0;
+    for (double v : C.ev3Sum) tot += v;
+    const double vol = C.v3Step * C.v3Step * C.v3Step;
+    if (axis) for (int i = 0; i < n3; ++i) axis[i] = C.v3Cells[i];
+    if (evdf3d)
+        for (size_t i = 0; i < C.ev3Sum.size(); ++i) evdf3d[i] = tot > 0 ? C.ev3Sum[i] / (tot * vol) : 0.0;
     return EKMC_OK;
 }
 

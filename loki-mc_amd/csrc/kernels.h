@@ -129,12 +129,14 @@ struct DevHist {
     unsigned long long* eedfPer;  // [n_cond * nPhases * nCells] or null
     unsigned long long* eadf;     // [n_cond * nCells * nCos] or null
     unsigned long long* evdf;     // [n_cond * nRad * nAx] or null
+    unsigned long long* evdf3;    // [n_cond * n3^3] (v_x, v_y, v_z) or null
     const double* grid;
     const int32_t* phaseRow;
     const int32_t* cyl;           // [n_cond] 1 = cylindrically symmetric
     const SlotRange* ranges;      // [nRanges] main-slot ranges, one block each
     int nRanges;
     int nCells, nCos, nRad, nAx, nPhases;
+    int n3;                       // cells per axis of the 3D EVDF (0: none)
     int histLds;                  // EEDF cells the flight kernel counts in LDS (0: in HBM)
 };
 

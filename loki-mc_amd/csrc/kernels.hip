@@ -1169,6 +1169,15 @@ __global__ __launch_bounds__(kBlock) void k_hist(DevState<V> s, const SlotRange*
             if (rx < h.nRad && rx >= 0 && az < h.nAx && az >= 0)
                 atomicAdd(&h.evdf[((size_t)c * h.nRad + rx) * h.nAx + az], 1ull);
         }
+        if (h.evdf3) {
+            // cells of width 2 v_max / n3 from -v_max on each axis; floor, so
+            // that a velocity below -v_max is not counted in the first cell
+            const double inv = h.n3 / (2.0 * maxSpeed);
+            const double fx = floor(((double)q.vx + maxSpeed) * inv), fy = floor(((double)q.vy + maxSpeed) * inv),
+                         fz = floor(((double)q.vz + maxSpeed) * inv);
+            if (fx >= 0 && fx < h.n3 && fy >= 0 && fy < h.n3 && fz >= 0 && fz < h.n3)
+                atomicAdd(&h.evdf3[(((size_t)c * h.n3 + (size_t)fx) * h.n3 + (size_t)fy) * h.n3 + (size_t)fz], 1ull);
+        }
     }
     if (!useLds) return;
     __syncthreads();

@@ -215,7 +215,7 @@ def workloads():
     out["cfg5_mix5_large"] = {
         "gases": ["mix5"], "fractions": [0.4, 0.2, 0.15, 0.15, 0.1],
         "gas_temperature_effect": "true", "ionization_operator": "equalSharing",
-        "initial_te_over_tg": 11.6045,
+        "initial_te_over_tg": 11.6045, "n_energy_cells": 1000, "evdf3d_cells": 128,
         "conditions": [_condition(300.0, angle=90.0, bn=500.0, tg=1000.0,
                                   n_e=100000000)]}
     return out

@@ -84,7 +84,7 @@ class Numerics(C.Structure):
                 ("sync_time_x_max_coll_freq", C.c_double), ("sync_over_sampling", C.c_int32),
                 ("max_sync_points", C.c_int32), ("seed", C.c_uint64),
                 ("work_chunk", C.c_int32), ("park_lanes", C.c_int32), ("env_coarsen", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("evdf3d_cells", C.c_int32)]
 
 
 class Results(C.Structure):
@@ -343,6 +343,8 @@ def workload(name, n_electrons=None, conditions=None):
         num.n_integration_phases = w["n_integration_phases"]
     if "n_energy_cells" in w:
         num.n_energy_cells = w["n_energy_cells"]
+    if "evdf3d_cells" in w:
+        num.evdf3d_cells = w["evdf3d_cells"]
     return mix, cs, num
 
 
@@ -411,6 +413,7 @@ def lib():
         L.ekmc_get_eedf_periodic.argtypes = [C.c_void_p, I32, pD, pD, pD]
         L.ekmc_get_swarm_periodic.argtypes = [C.c_void_p, I32, pD, pD, pD, pD, pD, pD]
         L.ekmc_get_evdf.argtypes = [C.c_void_p, I32, pD, pD, pD, pD, pD]
+        L.ekmc_get_evdf3d.argtypes = [C.c_void_p, I32, pD, pD]
         L.ekmc_get_series.argtypes = [C.c_void_p, I32, pI32, pD, pD, pD, pD, pD, pD, pD]
         L.ekmc_get_electrons.argtypes = [C.c_void_p, I32, pD, pD, pD, pD]
         L.ekmc_get_kernel_stats.argtypes = [C.c_void_p, _p(KernelStat), pI32]
@@ -523,6 +526,14 @@ class Engine:
         cells, f = np.zeros(n), np.zeros(n)
         check(lib().ekmc_get_eedf(self.h, cond, dptr(cells), dptr(f)))
         return cells, f
+
+    def evdf3d(self, cond=0):
+        """Cell centres of one axis and the normalised 3D EVDF [n, n, n] over
+        (v_x, v_y, v_z), for numerics.evdf3d_cells = n > 0."""
+        n = self.numerics.evdf3d_cells
+        cells, f = np.zeros(n), np.zeros(n * n * n)
+        check(lib().ekmc_get_evdf3d(self.h, cond, dptr(cells), dptr(f)))
+        return cells, f.reshape(n, n, n)
 
     def series(self, cond=0, capacity=1 << 20):
         n = C.c_int32(0)

@@ -53,7 +53,7 @@ def test_autotune_picks_a_candidate_and_the_pick_runs():
 def test_launch_parameters_out_of_range_are_rejected():
     mix, conds, num = ekmc.workload("cfg1_atomic40_dc", n_electrons=100)
     for field, bad in (("park_lanes", -1), ("park_lanes", 65), ("env_coarsen", -1), ("env_coarsen", 4),
-                       ("work_chunk", 65536)):
+                       ("work_chunk", 65536), ("work_chunk", -1), ("evdf3d_cells", -1), ("evdf3d_cells", 257)):
         n = ekmc.Numerics.from_buffer_copy(num)
         setattr(n, field, bad)
         with pytest.raises(ekmc.EngineError, match=field):

@@ -1,0 +1,68 @@
+"""The 3D electron velocity distribution (evdf3d_cells > 0; an extension: the
+reference's EVDF is the 2D one of cylindrically symmetric cases,
+BoltzmannMC.C:1562-1571) on a case whose answer is known.  Without a field,
+electrons that collide elastically with gas atoms moving at the gas
+temperature (the thermal-target branch, BoltzmannMC.C:916-1031,1125-1158)
+relax to the Maxwell velocity distribution: an isotropic Gaussian of variance
+k T_g / m_e per axis, whose second moment is the run's mean energy.
+"""
+import numpy as np
+import pytest
+
+import ekmc
+from helpers import condition, mixture
+
+pytestmark = pytest.mark.gpu
+
+
+def test_3d_evdf_of_a_relaxed_gas_is_the_maxwell_distribution():
+    from scipy.special import ndtr
+    T = 1000.0
+    mix = mixture([dict(type=0, elastic=True, energy=np.array([0.0, 1000.0]), sigma=np.array([1e-19, 1e-19]),
+                        mass_amu=20 * ekmc.EMASS / ekmc.AMU)])
+    cond = condition(en=1e-4, tg=T, n_e=100000)
+    num = ekmc.numerics_default()
+    num.gas_temperature_effect = ekmc.GASTEMP["true"]
+    num.initial_elec_temp_over_gas_temp = 1.0
+    num.n_integration_points = 4000
+    num.evdf3d_cells = 32
+    num.seed = 5
+    eng = ekmc.Engine(mix, [cond], num)
+    eng.solve()
+    cells, f = eng.evdf3d(0)
+    r = eng.results(0)
+    eng.close()
+    h = cells[1] - cells[0]
+    w = f * h ** 3
+    assert abs(w.sum() - 1.0) < 1e-12
+    assert np.allclose(cells, -cells[::-1], rtol=0, atol=1e-9 * h)
+    # per-axis variance of the run's temperature (mean energy = 3/2 m var / e)
+    var_run = 2.0 * r.mean_energy * ekmc.CHARGE / (3.0 * ekmc.EMASS)
+    var_gas = ekmc.BOLTZMANN * T / ekmc.EMASS
+    print("mean energy / (3/2 k T_g)", var_run / var_gas, "rel. error", r.mean_energy_error / r.mean_energy)
+    second = 0.0
+    for axis in range(3):
+        m = w.sum(axis=tuple(a for a in range(3) if a != axis))
+        mean = (m * cells).sum()
+        # Sheppard's correction: binning adds h^2 / 12 to the variance
+        var = (m * cells ** 2).sum() - mean ** 2 - h * h / 12.0
+        second += var + mean ** 2
+        F = np.cumsum(m)
+        dF = np.abs(F - ndtr((cells + 0.5 * h - mean) / np.sqrt(var))).max()
+        print("axis", axis, "mean / sigma %.2e" % (mean / np.sqrt(var_run)), "var / var_run %.5f" % (var / var_run),
+              "max |F - Gaussian| %.2e" % dF)
+        assert abs(mean) <= EVDF_TOL["mean"] * np.sqrt(var_run)
+        assert abs(var / var_run - 1.0) <= EVDF_TOL["var"]
+        assert dF <= EVDF_TOL["cdf"]
+    e3 = 0.5 * ekmc.EMASS * second / ekmc.CHARGE
+    print("mean energy from the 3D EVDF / the run's mean energy %.5f" % (e3 / r.mean_energy))
+    assert abs(e3 / r.mean_energy - 1.0) <= EVDF_TOL["var"]
+    # the run's temperature against the gas's: three standard errors of the
+    # run's mean energy (batch means) and 1e-3 for its relaxation
+    assert abs(r.mean_energy / (1.5 * ekmc.BOLTZMANN * T / ekmc.CHARGE) - 1.0) <= \
+        3.0 * r.mean_energy_error / r.mean_energy + 1e-3
+
+
+# measured on the MI355X: |mean| <= 9e-5 sigma, |var / var_run - 1| <= 2.4e-4,
+# max |F - Gaussian| <= 9.4e-5, second moment / mean energy = 1 within 1e-5
+EVDF_TOL = {"mean": 1e-3, "var": 2e-3, "cdf": 1e-3}

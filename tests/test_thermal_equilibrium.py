@@ -74,8 +74,11 @@ output:
 
 
 # (gas temperature, start temperature / T_g, integration points, bound on
-# |mean energy / (1.5 k T_g) - 1| beside 4 standard deviations)
+# |mean energy / (1.5 k T_g) - 1| beside 3 standard deviations)
 CASES = [(300.0, 10, "5E3", 0.005), (1000.0, 10, "5E3", 0.005), (300.0, 1, "4E4", 0.0015)]
+# measured 0.00095 (short runs) and 0.00016 (long run); a half-cell shift of the
+# EEDF grid gives about 0.02
+SHAPE_TOL = 0.003
 
 
 @pytest.mark.parametrize("T,start,points,bound", CASES)
@@ -91,14 +94,19 @@ def test_thermal_targets_relax_electrons_to_the_gas_temperature(tmp_path, T, sta
     m = re.search(r"Mean energy = (\S+) \(eV\) ; Rel. std: (\S+)%", sw)
     mean, rel = float(m.group(1)), float(m.group(2)) / 100.0
     print("T", T, "mean", mean, "expected", 1.5 * kT, "rel std", rel)
-    assert abs(mean - 1.5 * kT) <= 4.0 * rel * mean + bound * 1.5 * kT
+    assert abs(mean - 1.5 * kT) <= 3.0 * rel * mean + bound * 1.5 * kT
     # Maxwellian EEDF: the distribution of energies, F(e) = Int_0^e f sqrt(x) dx,
-    # against the Maxwell-Boltzmann one, a gamma(3/2, kT) distribution
+    # against the Maxwell-Boltzmann one, a gamma(3/2, kT) distribution.  The
+    # run's temperature error (checked above) moves F by about 0.46 times the
+    # relative error of the mean energy, so the shape is compared at the
+    # run's own temperature, 2/3 of its mean energy: a grid shifted by half a
+    # cell moves F by about 0.02
     from scipy.special import gammainc
     e = np.loadtxt(out / "eedf.txt", skiprows=1)
     de = e[1, 0] - e[0, 0]
     F = np.cumsum(e[:, 1] * np.sqrt(e[:, 0]) * de)
     edges = e[:, 0] + 0.5 * de
     dF = np.abs(F - gammainc(1.5, edges / kT))
-    print("max |F - F_Maxwell|", dF.max())
-    assert dF.max() < 0.03
+    dFs = np.abs(F - gammainc(1.5, edges / (mean / 1.5)))
+    print("max |F - F_Maxwell(T_g)|", dF.max(), "at the run's temperature", dFs.max())
+    assert dFs.max() < SHAPE_TOL
