@@ -110,7 +110,21 @@ struct DevWork {
     int recSize;                  // 3P + 8
     int nChunks;
     int nCond;
+    // Reduction groups, for batches with a condition of more than
+    // kFinishDirect work units: fixed ranges of at most kFinishGroup units
+    // of one condition, summed by one block each before the per-condition
+    // reduction (nGroups 0: the per-condition reduction reads the units).
+    const int32_t* groupCond;     // [nGroups] condition of the group
+    const int32_t* groupFirst;    // [nGroups] first work unit (global index)
+    const int32_t* groupCount;    // [nGroups] work units in the group
+    const int32_t* condGroup;     // [n_cond] first group of the condition
+    const int32_t* condGroups;    // [n_cond] groups of the condition
+    double* tailG;                // [nGroups * CT_SIZE]
+    double* momG;                 // [nGroups * kMomentRecord]
+    int nGroups;
 };
+constexpr int kFinishDirect = 4096;  // work units a condition reduces directly
+constexpr int kFinishGroup = 2048;   // work units per reduction group
 
 // A contiguous range of main slots of one condition (histogram blocks).
 struct SlotRange {

@@ -272,25 +272,54 @@ __device__ double block_max(double v, double* red) {
 }
 
 // Exclusive scan of `in[0,n)` (int32) into `out` (int64) by one block of
-// `blockDim.x` threads; returns the total in every thread.
+// `blockDim.x` threads (a multiple of the wave, at most 64 waves); returns the
+// total in every thread.  Tiles of 8 consecutive values per thread: a serial
+// scan in each thread, a shuffle scan of the thread totals in each wave, a
+// scan of the wave totals by the first wave, three barriers per tile (a
+// one-value-per-thread Hillis-Steele tile took twenty; a condition of 1e8
+// electrons has about 98,000 work units).  Integer sums: any order gives the
+// same offsets.  `lds` holds at least 2 kWave + 1 values.
 __device__ int64_t block_exclusive_scan(const int32_t* in, int64_t* out, int64_t n, int64_t* lds) {
-    const int B = blockDim.x;
+    constexpr int E = 8;
+    const int lane = lane_id(), wid = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    const int64_t tile = (int64_t)blockDim.x * E;
     int64_t running = 0;
-    for (int64_t base = 0; base < n; base += B) {
-        int64_t i = base + threadIdx.x;
-        int64_t v = i < n ? in[i] : 0;
-        lds[threadIdx.x] = v;
-        __syncthreads();
-        for (int o = 1; o < B; o <<= 1) {  // Hillis-Steele inclusive scan
-            int64_t add = threadIdx.x >= o ? lds[threadIdx.x - o] : 0;
-            __syncthreads();
-            lds[threadIdx.x] += add;
-            __syncthreads();
+    for (int64_t base = 0; base < n; base += tile) {
+        const int64_t i0 = base + (int64_t)threadIdx.x * E;
+        int64_t v[E], t = 0;
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            v[k] = i0 + k < n ? in[i0 + k] : 0;
+            t += v[k];
         }
-        if (i < n) out[i] = running + lds[threadIdx.x] - v;
-        int64_t tot = lds[B - 1];
+        int64_t incl = t;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int64_t u = __shfl_up(incl, o, kWave);
+            if (lane >= o) incl += u;
+        }
+        if (lane == kWave - 1) lds[wid] = incl;
         __syncthreads();
-        running += tot;
+        if (wid == 0) {
+            const int64_t wt = lane < nw ? lds[lane] : 0;
+            int64_t wi = wt;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int64_t u = __shfl_up(wi, o, kWave);
+                if (lane >= o) wi += u;
+            }
+            if (lane < nw) lds[kWave + lane] = wi - wt;
+            if (lane == kWave - 1) lds[2 * kWave] = wi;
+        }
+        __syncthreads();
+        int64_t pre = running + lds[kWave + wid] + incl - t;
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            if (i0 + k < n) out[i0 + k] = pre;
+            pre += v[k];
+        }
+        running += lds[2 * kWave];
+        __syncthreads();
     }
     return running;
 }
@@ -1197,21 +1226,20 @@ __global__ __launch_bounds__(kBlock) void k_hist(DevState<V> s, const SlotRange*
 // four waves in order.  The integer accumulators are converted and cleared
 // for the next interval.  Block 0 resets the flight-kernel work queues.
 constexpr int kFinishFields = CT_SIZE + kMomentRecord;
-__global__ __launch_bounds__(256) void k_finish(DevWork w, int P) {
-    __shared__ double red[kFinishFields][256 / kWave];
-    const int c = blockIdx.x;
-    if (c == 0 && threadIdx.x < 8) w.queue[threadIdx.x] = 0;
-    const CondParams cp = w.conds[c];
-    if (!cp.active) return;
-    double* out = w.summary + (size_t)c * w.summarySize;
-    const int R = w.recSize;
+// Fixed-order sums (maxima for the two maximum fields) over the records
+// [first, first + n) of a chunk-tail array and a moment array, by one
+// 256-thread block: each thread takes records threadIdx.x, + 256, ..., the
+// waves reduce by shuffles and thread f < kFinishFields adds the four wave
+// results of field f in wave order; the result is valid in those threads.
+__device__ double finish_reduce(const double* tsrc, const double* msrc, size_t first, int n,
+                                double (*red)[256 / kWave]) {
     double acc[kFinishFields];
 #pragma unroll
     for (int f = 0; f < kFinishFields; ++f) acc[f] = 0;
-    for (int k = threadIdx.x; k < cp.nChunks; k += blockDim.x) {
-        const size_t j = (size_t)cp.chunkBegin + k;
-        const double* t = w.tailC + j * CT_SIZE;
-        const double* m = w.momC + j * kMomentRecord;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const size_t j = first + k;
+        const double* t = tsrc + j * CT_SIZE;
+        const double* m = msrc + j * kMomentRecord;
 #pragma unroll
         for (int f = 0; f < CT_SIZE; ++f) acc[f] = f == CT_MAXE ? fmax(acc[f], t[f]) : acc[f] + t[f];
 #pragma unroll
@@ -1226,11 +1254,43 @@ __global__ __launch_bounds__(256) void k_finish(DevWork w, int P) {
         if (lane == 0) red[f][wid] = v;
     }
     __syncthreads();
+    double tot = 0;
     if (threadIdx.x < kFinishFields) {
         const int f = threadIdx.x;
         const bool isMax = f == CT_MAXE || f == kFinishFields - 1;
-        double tot = red[f][0];
+        tot = red[f][0];
         for (int i = 1; i < 256 / kWave; ++i) tot = isMax ? fmax(tot, red[f][i]) : tot + red[f][i];
+    }
+    return tot;
+}
+
+// First pass for conditions with many work units: group g's sums, in the
+// record layout of the chunk tails and moments.
+__global__ __launch_bounds__(256) void k_finish_group(DevWork w) {
+    __shared__ double red[kFinishFields][256 / kWave];
+    const int g = blockIdx.x;
+    if (!w.conds[w.groupCond[g]].active) return;
+    const double tot = finish_reduce(w.tailC, w.momC, (size_t)w.groupFirst[g], w.groupCount[g], red);
+    if (threadIdx.x < kFinishFields) {
+        const int f = threadIdx.x;
+        if (f < CT_SIZE) w.tailG[(size_t)g * CT_SIZE + f] = tot;
+        else w.momG[(size_t)g * kMomentRecord + (f - CT_SIZE)] = tot;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_finish(DevWork w, int P) {
+    __shared__ double red[kFinishFields][256 / kWave];
+    const int c = blockIdx.x;
+    if (c == 0 && threadIdx.x < 8) w.queue[threadIdx.x] = 0;
+    const CondParams cp = w.conds[c];
+    if (!cp.active) return;
+    double* out = w.summary + (size_t)c * w.summarySize;
+    const int R = w.recSize;
+    // the work units themselves, or the sums of their reduction groups
+    const double tot = w.nGroups ? finish_reduce(w.tailG, w.momG, (size_t)w.condGroup[c], w.condGroups[c], red)
+                                 : finish_reduce(w.tailC, w.momC, (size_t)cp.chunkBegin, cp.nChunks, red);
+    if (threadIdx.x < kFinishFields) {
+        const int f = threadIdx.x;
         if (f < CT_SIZE) out[3 * P + f] = tot;
         else out[R + (f - CT_SIZE)] = tot;
     }
@@ -1584,6 +1644,7 @@ hipError_t launch_hist(const DevState<V>& s, const DevHist& h, hipStream_t st) {
 
 hipError_t launch_finish(const DevWork& w, int P, hipStream_t st) {
     timer_start(K_REDUCE, st);
+    if (w.nGroups) hipLaunchKernelGGL(k_finish_group, dim3(w.nGroups), dim3(256), 0, st, w);
     hipLaunchKernelGGL(k_finish, dim3(w.nCond), dim3(256), 0, st, w, P);
     timer_stop(K_REDUCE, st);
     return hipGetLastError();

@@ -373,6 +373,38 @@ def test_one_interval_of_the_mixed_form_follows_the_oracle(case):
     eng.close()
 
 
+@pytest.mark.parametrize("prec", [ekmc.FP64, ekmc.MIXED])
+def test_sampled_moments_of_a_condition_with_many_work_units(prec):
+    """A batch with a condition of more than 4096 work units (300,000
+    electrons in units of 64) reduces its sums in two passes, groups of 2048
+    units and then each condition's groups.  The sampled moments of both
+    conditions (the large one and a small one beside it) equal the sums over
+    their electrons as downloaded: mean energy, position and velocity, and the
+    flux diffusion tensor <r v> - <r><v>."""
+    mix = _no_ionization_gas()
+    conds = [condition(en=120.0, n_e=1000), condition(en=120.0, angle=90.0, bn=300.0, n_e=300000)]
+    num = ekmc.numerics_default()
+    num.precision = prec
+    num.initial_elec_temp_over_gas_temp = 38.68
+    num.work_chunk = 64
+    num.seed = 77
+    eng = _engine(mix, conds, num)
+    eng.initialize()
+    eng.step(3)
+    for c in range(2):
+        pos, vel, e, _ = eng.electrons(c)
+        s = eng.series(c)
+        assert len(s["mean_energy"]) == 4  # the start and three intervals
+        r, v = pos.mean(0), vel.mean(0)
+        assert abs(s["mean_energy"][-1] / e.mean() - 1) <= 1e-12
+        np.testing.assert_allclose(s["mean_position"][-3:], r, rtol=1e-10, atol=1e-12 * np.abs(pos).max())
+        np.testing.assert_allclose(s["mean_velocity"][-3:], v, rtol=1e-10, atol=1e-12 * np.abs(vel).max())
+        flux = (pos[:, :, None] * vel[:, None, :]).mean(0) - np.outer(r, v)
+        np.testing.assert_allclose(s["flux_diffusion"][-9:], flux.ravel(), rtol=1e-8,
+                                   atol=1e-10 * np.abs(flux).max())
+    eng.close()
+
+
 def _short_numerics(seed, precision=ekmc.FP64):
     num = ekmc.numerics_default()
     num.precision = precision
