@@ -532,18 +532,18 @@ __device__ __forceinline__ int thermal_choose(const CondParams& c, const DevProc
         V over = (V)0.5 * (V)pr.proc[L].rMass * rel * rel / (V)kCharge / step;
         int k1 = (int)fmin(over, (V)(I - 1));
         int k2 = (int)fmin((V)(k1 + 1), (V)(I - 1));
-        V w1 = (V)k2 - over;
-        w1 = w1 < (V)0 ? (V)0 : (V)1;
-        V w2 = (V)1 - w1;
-        const V* cr1 = cumul + (size_t)k1 * P;
-        const V* cr2 = cumul + (size_t)k2 * P;
-        const V* i1 = interp + (size_t)k1 * P;
-        const V* i2 = interp + (size_t)k2 * P;
+        // the weights snap to {0, 1} (BoltzmannMC.C:959-967): row k1 has
+        // weight 1 unless k2 - over < 0.  A snapped pair picks one row, and
+        // 1 x + 0 x = x for the finite table values, so that row passed twice
+        // with weights (1, 0) gives the same numbers from one load per entry
+        const int kr = (V)k2 - over < (V)0 ? k2 : k1;
+        const V* cr = cumul + (size_t)kr * P;
+        const V* ir = interp + (size_t)kr * P;
         V ref = (V)0;
-        if (L > 0) ref = lerp2(w1, cr1[L - 1], w2, cr2[L - 1]);
-        V lim = prevSum + (lerp2(w1, cr1[R], w2, cr2[R]) - ref) * rel;
+        if (L > 0) ref = lerp2((V)1, cr[L - 1], (V)0, cr[L - 1]);
+        V lim = prevSum + (lerp2((V)1, cr[R], (V)0, cr[R]) - ref) * rel;
         if (randCS > lim) { prevSum = lim; continue; }
-        return channel_search(cr1, cr2, i1, i2, w1, w2, L, R, randCS, prevSum, ref, rel, true);
+        return channel_search(cr, cr, ir, ir, (V)1, (V)0, L, R, randCS, prevSum, ref, rel, true);
     }
     return -1;
 }

@@ -172,26 +172,43 @@ template <typename V> __device__ __forceinline__ lds_uint4* ring_segment(lds_uin
     return ring + ((record / kRingSeg) % ring_segs<V>()) * (rec_pieces<V>() * kRingSeg);
 }
 // Loads of main records [loaded, loaded + kRingSeg) of a chunk of `count`
-// into their segment: each direct-to-LDS load of the wave writes 1 KB, the
-// lower half-wave piece k of the 32 records and the upper half-wave piece
-// k + 1 (two pieces per instruction: mixed flight kernel 0.900 -> 0.883 ms
-// per interval on the 64-condition scan, FP64 1.477 -> 1.466 ms).  A fill
-// always holds record `loaded`, so it issues exactly rec_pieces / 2 loads.
-template <typename V>
+// into their segment.  PAIR: each direct-to-LDS load of the wave writes 1 KB,
+// the lower half-wave piece k of the 32 records and the upper half-wave piece
+// k + 1 (mixed flight kernel 0.900 -> 0.883 ms per interval on the
+// 64-condition scan, FP64 1.477 -> 1.466 ms); otherwise the lower half-wave
+// loads one piece per instruction.  A fill always holds record `loaded`, so
+// it issues exactly ring_loads<V, PAIR>() loads.
+template <typename V, bool PAIR> __host__ __device__ constexpr int ring_loads() {
+    return PAIR ? rec_pieces<V>() / 2 : rec_pieces<V>();
+}
+template <typename V, bool PAIR>
 __device__ __forceinline__ void ring_fill(const DevState<V>& s, int64_t first, int count, int loaded, lds_uint4* ring) {
     const int lane = lane_id();
-    static_assert(rec_pieces<V>() % 2 == 0, "pieces are loaded in pairs");
-    const int half = lane / kRingSeg;  // lanes 32-63 load the odd pieces
-    const int idx = loaded + (lane % kRingSeg);
     lds_uint4* seg = ring_segment<V>(ring, loaded);
-    if (idx < count) {
-        const char* srcA = (const char*)(s.ra + first + idx);
-        const char* srcB = (const char*)(s.rb + first + idx);
+    if constexpr (PAIR) {
+        static_assert(rec_pieces<V>() % 2 == 0, "pieces are loaded in pairs");
+        const int half = lane / kRingSeg;  // lanes 32-63 load the odd pieces
+        const int idx = loaded + (lane % kRingSeg);
+        if (idx < count) {
+            const char* srcA = (const char*)(s.ra + first + idx);
+            const char* srcB = (const char*)(s.rb + first + idx);
 #pragma unroll
-        for (int k = 0; k < rec_pieces<V>(); k += 2) {
-            const int piece = k + half;
-            const char* src = piece < rec_pieces_a<V>() ? srcA + 16 * piece : srcB + 16 * (piece - rec_pieces_a<V>());
-            __builtin_amdgcn_global_load_lds((global_void*)src, (lds_void*)(seg + k * kRingSeg), 16, 0, 0);
+            for (int k = 0; k < rec_pieces<V>(); k += 2) {
+                const int piece = k + half;
+                const char* src = piece < rec_pieces_a<V>() ? srcA + 16 * piece : srcB + 16 * (piece - rec_pieces_a<V>());
+                __builtin_amdgcn_global_load_lds((global_void*)src, (lds_void*)(seg + k * kRingSeg), 16, 0, 0);
+            }
+        }
+    } else {
+        const int idx = loaded + lane;
+        if (lane < kRingSeg && idx < count) {
+            const char* srcA = (const char*)(s.ra + first + idx);
+            const char* srcB = (const char*)(s.rb + first + idx);
+#pragma unroll
+            for (int k = 0; k < rec_pieces<V>(); ++k) {
+                const char* src = k < rec_pieces_a<V>() ? srcA + 16 * k : srcB + 16 * (k - rec_pieces_a<V>());
+                __builtin_amdgcn_global_load_lds((global_void*)src, (lds_void*)(seg + k * kRingSeg), 16, 0, 0);
+            }
         }
     }
 }
@@ -201,9 +218,9 @@ template <int N> __device__ __forceinline__ void wait_vmcnt() {
     __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
 }
 // Wait until a fill followed by `later` younger fills has landed (each fill
-// is rec_pieces / 2 loads; completion is in order).
-template <typename V> __device__ __forceinline__ void ring_wait(int later) {
-    constexpr int p = rec_pieces<V>() / 2;
+// is ring_loads<V, PAIR>() loads; completion is in order).
+template <typename V, bool PAIR> __device__ __forceinline__ void ring_wait(int later) {
+    constexpr int p = ring_loads<V, PAIR>();
     static_assert(ring_segs<V>() <= 4, "ring_wait covers at most 3 younger fills");
     if (later <= 0) wait_vmcnt<0>();
     else if (later == 1) wait_vmcnt<p>();
@@ -507,13 +524,17 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
         for (int i = lane; i < (h.nCells + 1) / 2; i += kWave) lhist[i] = 0;
     wave_lds_sync();
 
+    // two record pieces per ring load, except with the cylindrical histograms
+    // counted at load: there the pairs raised the flight kernel from 163 to
+    // 169 VGPRs, past the 168 of three waves per SIMD
+    constexpr bool kPairFill = HIST != HIST_LDS_CYL;
     const int count = ch.count;
     int nextMain = 0, nq = 0, nextNew = 0, holes = 0;
     int loaded = 0;  // main records whose ring loads are issued
     int landed = 0;  // ring fills known to have landed
     constexpr int kAhead = (ring_segs<V>() - 1) * kRingSeg;
     for (; loaded < count && loaded <= nextMain + kAhead; loaded += kRingSeg)
-        ring_fill<V>(s, ch.first, count, loaded, ring);
+        ring_fill<V, kPairFill>(s, ch.first, count, loaded, ring);
     Electron<V> e;
     bool have = false, isNew = false, parked = false;
     int64_t slot = 0;
@@ -535,7 +556,7 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
                 // fill they need, and the fills issued after it
                 const int need = (nextMain + takeMain - 1) / kRingSeg;
                 if (need >= landed) {
-                    ring_wait<V>(loaded / kRingSeg - 1 - need);
+                    ring_wait<V, kPairFill>(loaded / kRingSeg - 1 - need);
                     landed = need + 1;
                 }
             }
@@ -576,7 +597,7 @@ __device__ __forceinline__ void advance_chunk(const DevState<V>& s, const DevWor
         // 0.936 / 1.526 ms, the same filled before the flights 0.920 ms mixed)
         if (loaded + kRingSeg <= nextMain + kAhead)
             for (; loaded < count && loaded <= nextMain + kAhead; loaded += kRingSeg)
-                ring_fill<V>(s, ch.first, count, loaded, ring);
+                ring_fill<V, kPairFill>(s, ch.first, count, loaded, ring);
 
         bool done = false, dead = false, spawn = false;
         CollOut<V> o;
