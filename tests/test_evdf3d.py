@@ -17,6 +17,8 @@ pytestmark = pytest.mark.gpu
 # measured on the MI355X: |mean| <= 9e-5 sigma, |var / var_run - 1| <= 2.4e-4,
 # max |F - Gaussian| <= 9.4e-5, second moment / mean energy = 1 within 1e-5
 EVDF_TOL = {"mean": 1e-3, "var": 2e-3, "cdf": 1e-3}
+# measured in crossed fields: |3D EVDF mean velocity - flux drift| <= 2.4e-4 |W|
+DRIFT_TOL = 1e-3
 
 
 def test_3d_evdf_of_a_relaxed_gas_is_the_maxwell_distribution():
@@ -65,4 +67,37 @@ def test_3d_evdf_of_a_relaxed_gas_is_the_maxwell_distribution():
     # run's mean energy (batch means) and 1e-3 for its relaxation
     assert abs(r.mean_energy / (1.5 * ekmc.BOLTZMANN * T / ekmc.CHARGE) - 1.0) <= \
         3.0 * r.mean_energy_error / r.mean_energy + 1e-3
+
+
+
+def test_3d_evdf_first_moments_are_the_flux_drift_in_crossed_fields():
+    """In crossed fields (E along x, B along z: no cylindrical symmetry, the
+    case only the 3D EVDF describes) the mean velocity of the 3D EVDF is the
+    run's flux drift velocity, E x B drift included: both average the same
+    steady-state samples, one through the histogram and one through the
+    sampled moments."""
+    e = np.logspace(-3, 3, 400)
+    procs = [dict(type=0, elastic=True, energy=e, sigma=1e-20 * (0.3 + e / (1 + e)), mass_amu=20.0)]
+    for thr in (1.2, 4.0, 9.0):
+        ee = e[e >= thr]
+        procs.append(dict(type=0, threshold=thr, energy=ee, sigma=4e-21 * np.log(ee / thr) / (ee / thr),
+                          mass_amu=20.0))
+    cond = condition(en=120.0, angle=90.0, bn=300.0, n_e=100000)
+    num = ekmc.numerics_default()
+    num.initial_elec_temp_over_gas_temp = 38.68
+    num.n_integration_points = 2000
+    num.evdf3d_cells = 64
+    num.seed = 9
+    eng = ekmc.Engine(mixture(procs), [cond], num)
+    eng.solve()
+    cells, f = eng.evdf3d(0)
+    r = eng.results(0)
+    eng.close()
+    h = cells[1] - cells[0]
+    w = f * h ** 3
+    W = np.array(r.flux_drift_velocity)
+    mean_v = np.array([(w.sum(axis=tuple(a for a in range(3) if a != i)) * cells).sum() for i in range(3)])
+    scale = np.linalg.norm(W)
+    print("flux drift", W, "3D EVDF mean velocity", mean_v, "difference / |W|", (mean_v - W) / scale)
+    assert np.all(np.abs(mean_v - W) <= DRIFT_TOL * scale)
 
